@@ -1,0 +1,142 @@
+"""Benchmark of the LIMA_MD step on one MI355X.
+
+Default workload (the production configuration): 10 chains x 10 compounds x 128
+particles (12,800 chain particles) plus lattice solvent at 33 nm^-3 (~24k) in a
+10 nm periodic box, 16 LJ types, 2 fs, neighbour rebuild every 10 steps,
+Berendsen thermostat, FP32 pairs.  Synthetic seeded data, generated here.
+
+Prints ONE JSON line: ns/day (whole job), ms/step, the roofline of the dominant
+kernel (the all-pairs solvent<-chain LJ kernel, VALU FP32 bound), and the CPU
+baseline (the oracle with OpenMP on this host's cores, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_PAIR = 23          # algorithmic FP32 flops per directed LJ pair (force only)
+PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 vector/matrix peak (packed); unpacked FMA rate is half
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from quantom_amd import systems
+    if name == "production":
+        return [systems.production()], "production: 10 chains x 10 compounds x 128 + lattice solvent 33/nm^3, 10 nm, 16 types"
+    if name == "small":
+        return [systems.small()], "small: 1 chain x 2 compounds x 128 + 3000 solvent, 5 nm, 8 types"
+    if name == "large":
+        return [systems.large()], "large: 10 chains x 40 compounds x 128 + lattice solvent 33/nm^3, 18 nm, 16 types"
+    if name == "replicas":
+        return systems.replicas(256), "replicas: 256 systems, chain 128-1024, solvent 500-6000, 3-6 nm"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(systems_, params, budget_s):
+    from oracle.oracle import Oracle, num_threads
+    sysm = systems_[0]
+    o = Oracle(sysm, params, precision=32, fast=True)
+    o.step(1)                      # warm
+    steps, t = 0, 0.0
+    block = max(1, params.nlist_interval)
+    while steps < 4 * block and (t < budget_s or steps == 0):
+        t0 = time.perf_counter()
+        o.step(block)
+        t += time.perf_counter() - t0
+        steps += block
+    ms = 1000.0 * t / steps
+    return {"value": 172.8 * (sysm.dt / 2e-6) / ms, "unit": "ns/day", "ms_per_step": ms,
+            "cores": num_threads(True), "kind": "port",
+            "sample": f"{steps} steps ({steps // block} neighbour rebuilds) of the same system, "
+                      f"oracle/liblima_oracle_fast.so (g++ -O3 -fopenmp, FMA allowed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="production")
+    ap.add_argument("--accumulation", default="fp32", choices=["fp32", "fixed64"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
+              f"(multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    from quantom_amd.abi import ACCUM_FIXED64, ACCUM_FP32, RunParams
+    from quantom_amd.engine import Engine
+
+    t_gen = time.perf_counter()
+    syss, desc = workload(args.workload)
+    log(f"generated {args.workload} in {time.perf_counter() - t_gen:.1f}s")
+    n_chain = sum(s.n_chain for s in syss)
+    n_solv = sum(s.n_solvents for s in syss)
+    params = RunParams(nlist_interval=10, accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
+    eng = Engine(syss, params)
+    log(f"engine ready: chain {n_chain} solvent {n_solv} config {eng.config()}")
+    eng.step(args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.steps)          # synchronises at the end
+    t1 = time.perf_counter()
+    ms = 1000.0 * (t1 - t0) / args.steps
+    dt_fs = syss[0].dt * 1e6
+    ns_day = dt_fs * 86.4 / ms
+    log(f"timed {args.steps} steps: {ms:.4f} ms/step, {ns_day:.1f} ns/day")
+
+    kt = eng.time_kernels(args.kernel_reps)
+    log(f"kernel times (us): {kt}")
+    pairs_sc = sum(s.n_chain * s.n_solvents for s in syss)
+    flops = FLOPS_PER_PAIR * pairs_sc
+    t_sc = kt["solvent_chain"] * 1e-6
+    achieved = flops / t_sc / 1e12 if t_sc > 0 else 0.0
+    roof = {"bound": "mfma", "compute_unit": "VALU FP32 (no MFMA: pairwise LJ is not a contraction)",
+            "kernel": "solvent_chain_kernel", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(f"solvent_chain_kernel_{args.workload}"),
+            "flops_per_launch": flops, "avg_launch_us": kt["solvent_chain"],
+            "unpacked_fma_peak_frac": achieved / (PEAK_FP32_TFLOPS / 2)}
+
+    out = {
+        "metric": "ms/step & ns/day, 1 MI355X (single + 256-replica batch); adjoint grads/sec",
+        "value": ns_day, "unit": "ns/day", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.accumulation == "fp32" else "f32+i64acc", "data": "synthetic",
+        "config": {"workload": desc, "n_chain": n_chain, "n_solvents": n_solv, "n_systems": len(syss),
+                   "dt_fs": dt_fs, "nlist_interval": params.nlist_interval, "thermostat": "berendsen",
+                   "accumulation": args.accumulation, "launch": eng.config()},
+        "roofline": roof,
+        "kernels_us": kt,
+    }
+    if not args.no_cpu and len(syss) == 1:
+        cb = cpu_baseline(syss, params, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = cb["ms_per_step"] / ms
+        log(f"cpu baseline: {cb['ms_per_step']:.1f} ms/step on {cb['cores']} threads")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * lima.h — C ABI of the MI355X (gfx950) implementation of LIMA_MD's per-step path.
+ *
+ * The reference drives its step through the C++ class `Engine`
+ * (Quantom/Quantom/Engine.cuh:109-208): `Engine(Simulation*, ForceField)`
+ * (Engine.cu:9-38), `deviceMaster()` -> `step()` (Engine.cu:44-47, 400-451) and
+ * `hostMaster()` (Engine.cu:49-75).  State lives in `Simulation`/`Box`
+ * (Simulation.cuh:35-178) as AoS structs with compile-time capacities
+ * (Bodies.cuh:19-25, 169-472, 644-732; Constants.cuh:35-113).
+ *
+ * This header replaces that boundary with plain C: the system is described by
+ * flat arrays that restate the reference fields (capacities become runtime
+ * sizes), the engine owns all device memory, and every call returns a status
+ * code instead of calling exit().  One engine may hold several independent
+ * systems ("replicas"); a single system is a batch of one.
+ *
+ * Units are the reference's: nm, ns, kg/mol, J/mol; force is J/(mol*nm).
+ * An engine handle is single-threaded: calls on one handle must not overlap.
+ * Distinct handles are independent.
+ */
+#ifndef LIMA_H
+#define LIMA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIMA_ABI_VERSION 1
+
+/* ---- status codes (replace printf+exit in Engine.cuh:47-55, Bodies.cuh:215-245,
+ *      Engine.cu:383-386 and the device flag critical_error_encountered,
+ *      Simulation.cuh:61) ---------------------------------------------------- */
+enum {
+    LIMA_OK = 0,
+    LIMA_E_HIP = 1,            /* a HIP runtime call failed */
+    LIMA_E_CAPACITY = 2,       /* a neighbour list or staging buffer overflowed */
+    LIMA_E_CRITICAL_FORCE = 3, /* |F| > 2e12 on a chain particle (Engine.cu:1050-1053) */
+    LIMA_E_NAN = 4,            /* NaN solvent position (Engine.cu:1151-1154) */
+    LIMA_E_ARG = 5,            /* invalid argument or system description */
+    LIMA_E_UNSUPPORTED = 6     /* valid request this build does not implement */
+};
+
+/* ---- force field: ForceField{ParticleParameters[32]} (Forcefield.cuh:12-27).
+ *      Type 0 is the solvent type (ATOMTYPE_SOL, Forcefield.cuh:15). -------- */
+#define LIMA_MAX_ATOM_TYPES 32
+typedef struct lima_particle_params {
+    float mass;    /* kg/mol */
+    float sigma;   /* nm */
+    float epsilon; /* J/mol */
+} lima_particle_params;
+
+typedef struct lima_forcefield {
+    lima_particle_params particle_parameters[LIMA_MAX_ATOM_TYPES];
+} lima_forcefield;
+
+/* ---- bonded terms, compound-local (or bridge-local) particle indices
+ *      PairBond / AngleBond / DihedralBond (Bodies.cuh:61-113) -------------- */
+typedef struct lima_pair_bond {
+    float b0, kb;
+    uint32_t atom_indexes[2];
+} lima_pair_bond;
+
+typedef struct lima_angle_bond {
+    float theta_0, k_theta;
+    uint32_t atom_indexes[3];
+} lima_angle_bond;
+
+typedef struct lima_dihedral_bond {
+    float phi_0, k_phi;
+    uint32_t n;               /* periodicity (uint8_t in the reference) */
+    uint32_t atom_indexes[4];
+} lima_dihedral_bond;
+
+/* ---- one system: restates Box/Simulation/Compound/CompoundState/Solvent/
+ *      LJ_Ignores/CompoundBridgeCompact with runtime sizes.
+ *      All arrays are read during lima_create and never retained. ---------- */
+typedef struct lima_system {
+    float box_len;              /* BOX_LEN (Constants.cuh:35), per system */
+    double dt;                  /* Box::dt [ns] (Simulation.cuh:60, 152) */
+    float thermostat_target;    /* K; target temperature of this system */
+    lima_forcefield forcefield; /* per system (replicas may differ) */
+
+    /* compounds: compound c owns chain particles [compound_offset[c], compound_offset[c+1]) */
+    int32_t n_compounds;
+    const int32_t* compound_offset;     /* [n_compounds+1] */
+    const int32_t* key_particle_index;  /* [n_compounds] Compound::key_particle_index (Bodies.cuh:347) */
+    const float* confining_particle_sphere; /* [n_compounds] (Bodies.cuh:348) */
+    const float* chain_positions;       /* [n_chain*3] CompoundState::positions (Bodies.cuh:313) */
+    const float* chain_prev_positions;  /* [n_chain*3] Compound::prev_positions (Bodies.cuh:323) */
+    const uint8_t* chain_atom_types;    /* [n_chain]   Compound::atom_types (Bodies.cuh:325) */
+
+    /* compound bonded terms, concatenated per compound with offsets [n_compounds+1]
+       (Compound::singlebonds/anglebonds/dihedrals, Bodies.cuh:338-345) */
+    const int32_t* bond_offset;      const lima_pair_bond* bonds;
+    const int32_t* angle_offset;     const lima_angle_bond* angles;
+    const int32_t* dihedral_offset;  const lima_dihedral_bond* dihedrals;
+
+    /* LJ exclusions per chain particle: entries [ignore_offset[i], ignore_offset[i+1])
+       hold (compound id, compound-local id) pairs (LJ_Ignores, Bodies.cuh:169-198) */
+    const int32_t* ignore_offset;    /* [n_chain+1] */
+    const int32_t* ignore_compound;
+    const int32_t* ignore_local;
+
+    /* bridges: bonded terms spanning compounds (CompoundBridgeCompact, Bodies.cuh:652-725).
+       Bridge b references particles [bridge_particle_offset[b], ...+1) by
+       (compound id, compound-local id); its terms use bridge-local indices. */
+    int32_t n_bridges;
+    const int32_t* bridge_particle_offset;   /* [n_bridges+1] */
+    const int32_t* bridge_particle_compound;
+    const int32_t* bridge_particle_local;
+    const int32_t* bridge_bond_offset;      const lima_pair_bond* bridge_bonds;
+    const int32_t* bridge_angle_offset;     const lima_angle_bond* bridge_angles;
+    const int32_t* bridge_dihedral_offset;  const lima_dihedral_bond* bridge_dihedrals;
+
+    /* solvents: Solvent{pos, pos_tsub1} (Bodies.cuh:19-25) */
+    int32_t n_solvents;
+    const float* solvent_positions;      /* [n_solvents*3] */
+    const float* solvent_prev_positions; /* [n_solvents*3] */
+} lima_system;
+
+/* ---- run parameters (compile-time constants of Constants.cuh made runtime) */
+enum { LIMA_THERMOSTAT_NONE = 0, LIMA_THERMOSTAT_BERENDSEN = 1 };
+enum { LIMA_ACCUM_FP32 = 0, LIMA_ACCUM_FIXED64 = 1 };
+
+typedef struct lima_run_params {
+    float cutoff;               /* CUTOFF (Constants.cuh:52); list radii are 2*cutoff (+spheres) */
+    int32_t nlist_interval;     /* STEPS_PER_NLIST_UPDATE (Constants.cuh:71) */
+    int32_t thermostat;         /* LIMA_THERMOSTAT_* */
+    double thermostat_tau;      /* ns, Berendsen coupling time */
+    int32_t accumulation;       /* LIMA_ACCUM_* */
+    int32_t enable_solvents;    /* ENABLE_SOLVENTS (Constants.cuh:13) */
+    int32_t enable_dihedrals;   /* ENABLE_DIHEDRALS (Constants.cuh:25) */
+    int32_t max_compound_neighbors;  /* NEIGHBORLIST_MAX_COMPOUNDS (Constants.cuh:41) */
+    int32_t max_compound_solvents;   /* NEIGHBORLIST_MAX_SOLVENTS for compound lists (Constants.cuh:42) */
+    int32_t max_solvent_neighbors;   /* NEIGHBORLIST_MAX_SOLVENTS for solvent lists (Constants.cuh:42) */
+    int32_t device;             /* HIP device ordinal */
+} lima_run_params;
+
+/* FP64 energy report (Analyzer::analyzeEnergy, Analyzer.cu:137-207, made in-step) */
+typedef struct lima_energies {
+    double bond, angle, dihedral;       /* reference bonded energies, accumulated */
+    double lj_intra, lj_inter, lj_compound_solvent, lj_solvent_compound, lj_solvent_solvent;
+                                         /* 1/2 * sum over directed pairs of 4*eps*(s^2-s) */
+    double potential;                    /* sum of all of the above */
+    double kinetic;                      /* sum 1/2 m |x - x_prev|^2 / dt^2 */
+    double temperature;                  /* 2 KE / (3 N R), R = 8.3145 */
+    double virial;                       /* sum over directed pairs of (r_j - r_i) . F_i / 2 */
+    double thermostat_scalar;            /* lambda applied on the most recent step */
+} lima_energies;
+
+typedef struct lima_engine lima_engine;
+
+/* library */
+const char* lima_version(void);
+int lima_device_count(int* count);
+
+/* Engine(Simulation*, ForceField) (Engine.cu:9-38): copies the systems to
+   the device and builds the neighbour lists (handleNLISTS(force), Engine.cu:35). */
+int lima_create(const lima_system* systems, int32_t n_systems,
+                const lima_run_params* params, lima_engine** out);
+void lima_destroy(lima_engine* engine);
+const char* lima_last_error(const lima_engine* engine);
+
+/* deviceMaster()+hostMaster() repeated n_steps times (Engine.cu:44-75, 400-451):
+   bridge, compound and solvent forces, integration, PBC, thermostat and the
+   scheduled neighbour-list rebuilds, all on the device.  Synchronises once at
+   the end and reports device error flags. */
+int lima_step(lima_engine* engine, int32_t n_steps);
+/* Queue n_steps without synchronising (errors surface at the next lima_sync). */
+int lima_step_async(lima_engine* engine, int32_t n_steps);
+int lima_sync(lima_engine* engine);
+
+/* Engine::updateNeighborLists (Engine.cu:146-287), synchronous and on the device */
+int lima_rebuild_neighbors(lima_engine* engine);
+
+/* state in lima_system layout, all systems concatenated (chain then solvent per system order) */
+int lima_get_state(lima_engine* engine, float* chain_positions, float* chain_prev_positions,
+                   float* solvent_positions, float* solvent_prev_positions);
+int lima_set_state(lima_engine* engine, const float* chain_positions, const float* chain_prev_positions,
+                   const float* solvent_positions, const float* solvent_prev_positions);
+/* total force used to integrate each particle on the most recent step */
+int lima_get_forces(lima_engine* engine, float* chain_forces, float* solvent_forces);
+/* forces on the current state without advancing it (the step's force pass only) */
+int lima_compute_forces(lima_engine* engine, float* chain_forces, float* solvent_forces);
+
+/* FP64 energies of the current state, one record per system */
+int lima_get_energies(lima_engine* engine, lima_energies* out);
+
+/* neighbour lists (NeighborList, Bodies.cuh:203-292) as global ids within a system.
+   counts: [n_compounds] / [n_solvents]; ids: stride = the matching capacity. */
+int lima_get_neighbor_counts(lima_engine* engine, int32_t* compound_compound_count,
+                             int32_t* compound_solvent_count, int32_t* solvent_solvent_count);
+int lima_get_neighbors(lima_engine* engine, int32_t* compound_compound_ids,
+                       int32_t* compound_solvent_ids, int32_t* solvent_solvent_ids);
+int lima_set_neighbors(lima_engine* engine,
+                       const int32_t* compound_compound_count, const int32_t* compound_compound_ids,
+                       const int32_t* compound_solvent_count, const int32_t* compound_solvent_ids,
+                       const int32_t* solvent_solvent_count, const int32_t* solvent_solvent_ids);
+
+/* Engine::timings (Engine.cuh:123): [0] device step time us, [1] rebuild time us,
+   [2] host time us, [3] steps run, [4] rebuilds run */
+int lima_get_timings(lima_engine* engine, double* out5);
+/* Average device time in microseconds of each kernel of the step, each launched
+   `reps` times alone on the engine's stream between HIP events, on the current
+   state: [0] solvent<-chain all-pairs, [1] compound listed LJ, [2] compound
+   bonded+intra+integrate (forces only), [3] solvent listed+integrate (forces
+   only), [4] bridge, [5] full neighbour-list rebuild. */
+int lima_time_kernels(lima_engine* engine, int32_t reps, double* out6);
+/* per-kernel launch shape chosen for this engine (see lima_autotune) */
+int lima_get_config(lima_engine* engine, int32_t* out8);
+int lima_autotune(lima_engine* engine, int32_t trial_steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIMA_H */

@@ -1,0 +1,156 @@
+// device_common.h — shared device-side types and arithmetic for the gfx950 kernels.
+//
+// The FP32 helpers restate reference arithmetic where results must be
+// bit-identical to the CPU restatement (neighbour-list distances, PBC,
+// re-imaging, integration): they use explicitly rounded intrinsics so that no
+// FMA contraction or reassociation can change the result.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LIMA_DEV __device__ __forceinline__
+
+namespace lima {
+
+constexpr int kMaxTypes = 32;
+constexpr int kCompoundBlock = 128;      // max particles per compound (2 wavefronts)
+constexpr int kSolventChunk = 256;       // solvent-list entries per compound work item
+constexpr int kFixedShift = 21;          // int64 fixed point: F_fixed = rint(F * 2^21)
+constexpr int kMaxBondsPerCompound = 256;
+constexpr int kMaxAnglesPerCompound = 384;
+constexpr int kMaxDihedralsPerCompound = 512;
+
+// device error flags (replace exit()/critical_error_encountered)
+enum : uint32_t {
+    kErrCapacity = 1u,
+    kErrCriticalForce = 2u,
+    kErrNaN = 4u,
+};
+
+struct ReplicaInfo {
+    float L, half;
+    double dt;
+    double tau;
+    float T0;
+    int comp_begin, comp_end;     // global compound ids
+    int chain_begin, chain_end;   // global chain particle ids
+    int solv_begin, solv_end;     // global solvent ids
+    int stile_begin, stile_end;   // solvent integrate tiles (KE partials)
+    int chunk_count;              // solvent<-chain compound chunks of this replica
+    int cell_begin;               // first cell of this replica's solvent grid
+    int ncell;                    // cells per dimension
+    float cell_inv;               // ncell / L
+    int pad0;
+};
+
+// work item of the all-pairs solvent<-chain kernel
+struct SCWork {
+    int replica;
+    int solv_begin, solv_end;
+    int comp_begin, comp_end;     // compound range of this chunk
+    int chunk;                    // partial slot index within the replica
+    int pad0, pad1;
+};
+
+// ---- exact FP32 arithmetic (no contraction) --------------------------------
+LIMA_DEV float fadd(float a, float b) { return __fadd_rn(a, b); }
+LIMA_DEV float fsub(float a, float b) { return __fsub_rn(a, b); }
+LIMA_DEV float fmul(float a, float b) { return __fmul_rn(a, b); }
+
+// Float3::len (QuantomTypes.cuh:60): sqrtf((x*x + y*y) + z*z), correctly rounded
+LIMA_DEV float exact_len(float x, float y, float z) {
+    return __fsqrt_rn(fadd(fadd(fmul(x, x), fmul(y, y)), fmul(z, z)));
+}
+
+// LIMAENG::applyHyperpos (Engine.cuh:24-30) on one coordinate
+LIMA_DEV float hyper1(float s, float m, float L, float half) {
+    m = fadd(m, fmul(L, (float)(fsub(s, m) > half)));
+    m = fsub(m, fmul(L, (float)(fsub(s, m) < -half)));
+    return m;
+}
+LIMA_DEV void hyperpos(float sx, float sy, float sz, float& mx, float& my, float& mz, float L, float half) {
+    mx = hyper1(sx, mx, L, half);
+    my = hyper1(sy, my, L, half);
+    mz = hyper1(sz, mz, L, half);
+}
+
+// applyPBC (Engine.cu:461-466) on one coordinate
+LIMA_DEV float pbc1(float p, float L) {
+    p = fadd(p, fmul(L, (float)(p < 0.0f)));
+    p = fsub(p, fmul(L, (float)(p > L)));
+    return p;
+}
+
+// Engine::neighborWithinCutoff (Engine.cu:146-151)
+LIMA_DEV bool within(float ax, float ay, float az, float bx, float by, float bz, float offset, float cutoff,
+                     float L, float half) {
+    hyperpos(ax, ay, az, bx, by, bz, L, half);
+    float d = exact_len(fsub(ax, bx), fsub(ay, by), fsub(az, bz));
+    return d < fadd(cutoff, offset);
+}
+
+// integratePosition (Engine.cu:849-869); returns the displacement after rescale.
+LIMA_DEV void integrate(float& px, float& py, float& pz, float& qx, float& qy, float& qz, float fx, float fy,
+                        float fz, double mts, float lambda, float L, float half, float& dx, float& dy, float& dz) {
+    float tx = px, ty = py, tz = pz;
+    hyperpos(px, py, pz, qx, qy, qz, L, half);
+    float nx = fadd(fsub(fmul(px, 2.0f), qx), (float)((double)fx * mts));
+    float ny = fadd(fsub(fmul(py, 2.0f), qy), (float)((double)fy * mts));
+    float nz = fadd(fsub(fmul(pz, 2.0f), qz), (float)((double)fz * mts));
+    qx = tx; qy = ty; qz = tz;
+    float ex = fsub(nx, qx), ey = fsub(ny, qy), ez = fsub(nz, qz);
+    px = fadd(qx, fmul(ex, lambda));
+    py = fadd(qy, fmul(ey, lambda));
+    pz = fadd(qz, fmul(ez, lambda));
+    dx = fsub(px, qx); dy = fsub(py, qy); dz = fsub(pz, qz);
+}
+
+// ---- LJ pair (calcLJForce, Engine.cu:475-522), fast form -------------------
+// sig2 = sigma*sigma, e24 = 24*epsilon, both precomputed per type pair with the
+// reference's float rounding.  1/r^2 uses the hardware reciprocal (1 ulp)
+// instead of two IEEE divisions.
+LIMA_DEV void lj_acc(float px, float py, float pz, float qx, float qy, float qz, float sig2, float e24, float& fx,
+                     float& fy, float& fz) {
+    float dx = qx - px, dy = qy - py, dz = qz - pz;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float inv = __builtin_amdgcn_rcpf(r2);
+    float s = sig2 * inv;
+    float s3 = s * s * s;
+    float fs = e24 * s3 * inv * (1.0f - 2.0f * s3);
+    fx += dx * fs;
+    fy += dy * fs;
+    fz += dz * fs;
+}
+
+// same pair with FP64 energy 4*eps*(s^2 - s) and virial d.F accumulation
+LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, float qz, float sig2, float e24,
+                            float& fx, float& fy, float& fz, double& e, double& w) {
+    float dx = qx - px, dy = qy - py, dz = qz - pz;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float inv = __builtin_amdgcn_rcpf(r2);
+    float s = sig2 * inv;
+    float s3 = s * s * s;
+    float fs = e24 * s3 * inv * (1.0f - 2.0f * s3);
+    fx += dx * fs;
+    fy += dy * fs;
+    fz += dz * fs;
+    double sd = (double)sig2 / (double)r2;
+    double s6 = sd * sd * sd;
+    e += (double)e24 / 6.0 * (s6 * s6 - s6);
+    w += (double)r2 * (double)fs;
+}
+
+LIMA_DEV long long to_fixed(float v) {
+    return __float2ll_rn(v * (float)(1 << kFixedShift));
+}
+LIMA_DEV float from_fixed(long long v) {
+    return (float)((double)v * (1.0 / (double)(1 << kFixedShift)));
+}
+
+// wavefront reduction of a double (wave64)
+LIMA_DEV double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+}  // namespace lima

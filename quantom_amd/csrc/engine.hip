@@ -1,0 +1,1006 @@
+// engine.hip — host engine and C ABI (include/lima.h) of the gfx950 LIMA_MD step.
+//
+// Replaces the reference's Engine (Engine.cuh:109-208; Engine.cu:9-132,
+// 400-451): the engine owns all device memory (hipMalloc, no managed memory),
+// runs the whole step — forces, integration, PBC, thermostat and the scheduled
+// neighbour-list rebuild — on one HIP stream, and replays each rebuild cycle as
+// a captured hipGraph.  Errors are status codes plus a device error word read
+// once per lima_step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lima.h"
+#include "kernels.h"
+
+using namespace lima;
+
+namespace {
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            throw LimaError(LIMA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+        }                                                                                     \
+    } while (0)
+
+struct LimaError {
+    int code;
+    std::string msg;
+    LimaError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+template <typename T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        free();
+        n = count;
+        if (count) HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const std::vector<T>& v) {
+        alloc(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void zero() {
+        if (p) HIP_TRY(hipMemset(p, 0, n * sizeof(T)));
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct LaunchConfig {
+    int spt = 2;                  // solvents per thread in solvent_chain_kernel (1, 2, 4)
+    int chunk_particles = 1024;   // target chain particles per solvent<-chain work item
+    int listed_grid = 0;          // grid of compound_listed_kernel (0 = auto)
+    int use_graph = 1;
+};
+
+}  // namespace
+
+struct lima_engine {
+    // ---- configuration ----
+    lima_run_params params{};
+    LaunchConfig cfg;
+    int R = 0;
+    std::vector<ReplicaInfo> reps_h;
+    int NC = 0, NCH = 0, NS = 0, NB = 0;
+    std::vector<int> sys_chain_begin, sys_solv_begin, sys_comp_begin;  // per system, into the concatenation
+    int n_cells = 0, n_stiles = 0;
+    int kmax = 0;                 // max solvent<-chain chunks in any replica
+    size_t max_slots = 0;
+    std::vector<SCWork> scwork_h;
+    int device = 0;
+
+    // ---- device buffers ----
+    DBuf<ReplicaInfo> reps;
+    DBuf<float4> cpos[2], cprev, cforce, fbridge;
+    DBuf<uint32_t> intra_mask;
+    DBuf<int> xoff;
+    DBuf<int2> xent;
+    DBuf<int> comp_off, comp_rep, comp_key;
+    DBuf<float> comp_sphere;
+    DBuf<int> bond_off, angle_off, dih_off, inc_off;
+    DBuf<BondD> bonds;
+    DBuf<AngleD> angles;
+    DBuf<DihD> dihs;
+    DBuf<uint32_t> inc;
+    DBuf<int> br_off, br_glob, br_rep, brb_off, bra_off, brd_off;
+    DBuf<BondD> brb;
+    DBuf<AngleD> bra;
+    DBuf<DihD> brd;
+    DBuf<float4> spos[2], sprev, sforce;
+    DBuf<int> solv_rep;
+    DBuf<int4> stiles;
+    DBuf<float> mass;
+    DBuf<float2> tab_sc, tab_cs, tab_inter, tab_intra, tab_ss;
+    DBuf<int> cc_cnt, cc_ids, cc_old_cnt, cc_old_ids, cs_cnt, cs_ids, ss_cnt, ss_ids;
+    DBuf<int> cw_off, cw_total;
+    DBuf<float4> partC, partS;
+    DBuf<I3> partC_fx, partS_fx;
+    DBuf<SCWork> scwork;
+    DBuf<float> lambda, lambda_applied;
+    DBuf<double> ke_comp, ke_stile, temperature, energy;
+    DBuf<uint32_t> err;
+    DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell;
+
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+
+    // ---- run state ----
+    int cur = 0;                  // ping-pong parity of the current state
+    long long step = 0;
+    long long since_rebuild = 0;
+    double t_device_us = 0, t_rebuild_us = 0, t_host_us = 0;
+    long long n_rebuilds = 0;
+    std::string last_error;
+
+    ~lima_engine() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    StepArgs step_args(int parity) {
+        StepArgs a{};
+        a.reps = reps.p;
+        a.n_replicas = R;
+        a.cpos = cpos[parity].p;
+        a.cpos_next = cpos[parity ^ 1].p;
+        a.cprev = cprev.p;
+        a.cforce = cforce.p;
+        a.fbridge = fbridge.p;
+        a.intra_mask = intra_mask.p;
+        a.xoff = xoff.p;
+        a.xent = xent.p;
+        a.n_comp = NC;
+        a.comp_off = comp_off.p;
+        a.comp_rep = comp_rep.p;
+        a.comp_key = comp_key.p;
+        a.comp_sphere = comp_sphere.p;
+        a.bond_off = bond_off.p; a.bonds = bonds.p;
+        a.angle_off = angle_off.p; a.angles = angles.p;
+        a.dih_off = dih_off.p; a.dihs = dihs.p;
+        a.inc_off = inc_off.p; a.inc = inc.p;
+        a.n_bridges = NB;
+        a.br_off = br_off.p; a.br_glob = br_glob.p; a.br_rep = br_rep.p;
+        a.brb_off = brb_off.p; a.brb = brb.p;
+        a.bra_off = bra_off.p; a.bra = bra.p;
+        a.brd_off = brd_off.p; a.brd = brd.p;
+        a.n_solv = NS;
+        a.spos = spos[parity].p;
+        a.spos_next = spos[parity ^ 1].p;
+        a.sprev = sprev.p;
+        a.sforce = sforce.p;
+        a.solv_rep = solv_rep.p;
+        a.stiles = stiles.p;
+        a.mass = mass.p;
+        a.tab_sc = tab_sc.p; a.tab_cs = tab_cs.p; a.tab_inter = tab_inter.p; a.tab_intra = tab_intra.p; a.tab_ss = tab_ss.p;
+        a.cap_cc = params.max_compound_neighbors;
+        a.cap_cs = params.max_compound_solvents;
+        a.cap_ss = params.max_solvent_neighbors;
+        a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p;
+        a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
+        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
+        a.cw_off = cw_off.p; a.cw_total = cw_total.p;
+        a.partC = partC.p; a.partC_fx = partC_fx.p;
+        a.partS = partS.p; a.partS_fx = partS_fx.p;
+        a.scwork = scwork.p; a.n_scwork = (int)scwork_h.size();
+        a.lambda = lambda.p;
+        a.ke_comp = ke_comp.p;
+        a.ke_stile = ke_stile.p;
+        a.energy = energy.p;
+        a.err = err.p;
+        a.enable_solvents = params.enable_solvents;
+        a.enable_dihedrals = params.enable_dihedrals;
+        return a;
+    }
+
+    NlistArgs nlist_args(int parity) {
+        NlistArgs a{};
+        a.reps = reps.p;
+        a.cpos = cpos[parity].p;
+        a.spos = spos[parity].p;
+        a.n_comp = NC;
+        a.n_solv = NS;
+        a.comp_off = comp_off.p; a.comp_rep = comp_rep.p; a.comp_key = comp_key.p; a.comp_sphere = comp_sphere.p;
+        a.solv_rep = solv_rep.p;
+        a.cutoff = params.cutoff;
+        a.cap_cc = params.max_compound_neighbors;
+        a.cap_cs = params.max_compound_solvents;
+        a.cap_ss = params.max_solvent_neighbors;
+        a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p; a.cc_old_cnt = cc_old_cnt.p; a.cc_old_ids = cc_old_ids.p;
+        a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
+        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
+        a.n_cells = n_cells;
+        a.cell_cnt = cell_cnt.p; a.cell_start = cell_start.p; a.cell_cursor = cell_cursor.p;
+        a.cell_ids = cell_ids.p; a.solv_cell = solv_cell.p;
+        a.cw_off = cw_off.p; a.cw_total = cw_total.p;
+        a.err = err.p;
+        a.enable_solvents = params.enable_solvents;
+        return a;
+    }
+
+    // ---- launches ----
+    template <bool ADVANCE, bool FIXED, bool ENERGY> void launch_forces(int parity) {
+        StepArgs a = step_args(parity);
+        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, stream>>>(a);
+        if (params.enable_solvents && !scwork_h.empty()) {
+            const int nw = (int)scwork_h.size();
+            switch (cfg.spt) {
+                case 1: solvent_chain_kernel<1, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
+                case 4: solvent_chain_kernel<4, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
+                default: solvent_chain_kernel<2, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
+            }
+        }
+        if (NC > 0) {
+            const int grid = cfg.listed_grid > 0 ? cfg.listed_grid : 4096;
+            compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, stream>>>(a);
+            compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, stream>>>(a);
+        }
+        if (params.enable_solvents && NS > 0)
+            solvent_integrate_kernel<ADVANCE, FIXED, ENERGY><<<n_stiles, 256, 0, stream>>>(a);
+    }
+
+    void launch_step(int parity) {
+        if (params.accumulation == LIMA_ACCUM_FIXED64) launch_forces<true, true, false>(parity);
+        else launch_forces<true, false, false>(parity);
+        // with solvents disabled the solvent state is carried across the ping-pong unchanged
+        if (!params.enable_solvents && NS > 0)
+            HIP_TRY(hipMemcpyAsync(spos[parity ^ 1].p, spos[parity].p, NS * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
+                                                temperature.p, params.thermostat == LIMA_THERMOSTAT_BERENDSEN ? 1 : 0);
+    }
+
+    void launch_rebuild(int parity) {
+        NlistArgs a = nlist_args(parity);
+        if (NC > 0) {
+            nl_copy_cc_kernel<<<NC, 64, 0, stream>>>(a);
+            nl_compound_compound_kernel<<<NC, 64, 0, stream>>>(a);
+            if (params.enable_solvents && NS > 0) nl_compound_solvent_kernel<<<NC, 256, 0, stream>>>(a);
+        }
+        if (params.enable_solvents && NS > 0) {
+            nl_cell_zero_kernel<<<(n_cells + 255) / 256, 256, 0, stream>>>(cell_cnt.p, n_cells);
+            nl_cell_count_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
+            nl_cell_scan_kernel<<<1, 1024, 0, stream>>>(a);
+            nl_cell_scatter_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
+            nl_cell_sort_kernel<<<n_cells, 256, 0, stream>>>(a);
+            nl_solvent_solvent_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
+        }
+        if (NC > 0) nl_work_kernel<<<1, 1024, 0, stream>>>(a);
+    }
+
+    int cycle_steps() const {
+        const int iv = params.nlist_interval > 0 ? params.nlist_interval : 1;
+        return (iv % 2 == 0) ? iv : 2 * iv;
+    }
+
+    // one step plus the rebuild it schedules (Engine::hostMaster, Engine.cu:49-75)
+    void enqueue_one() {
+        launch_step(cur);
+        cur ^= 1;
+        step++;
+        since_rebuild++;
+        if (params.nlist_interval > 0 && since_rebuild >= params.nlist_interval) {
+            launch_rebuild(cur);
+            since_rebuild = 0;
+            n_rebuilds++;
+        }
+    }
+
+    void build_graph() {
+        if (graph_exec) { HIP_TRY(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
+        const int n = cycle_steps();
+        // capture starting from parity 0 and a fresh rebuild cycle
+        const int save_cur = cur;
+        const long long save_step = step, save_since = since_rebuild, save_nr = n_rebuilds;
+        cur = 0;
+        since_rebuild = 0;
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < n; i++) enqueue_one();
+        HIP_TRY(hipStreamEndCapture(stream, &g));
+        HIP_TRY(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+        HIP_TRY(hipGraphDestroy(g));
+        cur = save_cur;
+        step = save_step;
+        since_rebuild = save_since;
+        n_rebuilds = save_nr;
+        graph_steps = n;
+    }
+
+    void enqueue_steps(int n) {
+        const int cyc = cycle_steps();
+        while (n > 0) {
+            if (cfg.use_graph && params.nlist_interval > 0 && cur == 0 && since_rebuild == 0 && n >= cyc) {
+                if (!graph_exec) build_graph();
+                HIP_TRY(hipGraphLaunch(graph_exec, stream));
+                step += cyc;
+                n_rebuilds += cyc / params.nlist_interval;
+                n -= cyc;
+                continue;
+            }
+            enqueue_one();
+            n--;
+        }
+    }
+
+    int check_errors() {
+        uint32_t e[4];
+        HIP_TRY(hipMemcpy(e, err.p, sizeof(e), hipMemcpyDeviceToHost));
+        if (e[0] == 0) return LIMA_OK;
+        HIP_TRY(hipMemset(err.p, 0, 4 * sizeof(uint32_t)));
+        char buf[256];
+        if (e[0] & kErrCapacity) {
+            snprintf(buf, sizeof(buf), "neighbour list capacity exceeded (needed: compound-compound %u, compound-solvent %u, solvent %u)",
+                     e[1], e[2], e[3]);
+            last_error = buf;
+            return LIMA_E_CAPACITY;
+        }
+        if (e[0] & kErrCriticalForce) { last_error = "critical force |F| > 2e12 on a chain particle"; return LIMA_E_CRITICAL_FORCE; }
+        if (e[0] & kErrNaN) { last_error = "NaN solvent position"; return LIMA_E_NAN; }
+        last_error = "unknown device error";
+        return LIMA_E_HIP;
+    }
+};
+
+namespace {
+
+void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
+    const lima_run_params& P = E->params;
+    if (nsys <= 0) throw LimaError(LIMA_E_ARG, "n_systems must be >= 1");
+    if (!(P.cutoff > 0)) throw LimaError(LIMA_E_ARG, "cutoff must be > 0");
+    if (P.max_compound_neighbors <= 0 || P.max_compound_solvents <= 0 || P.max_solvent_neighbors <= 0)
+        throw LimaError(LIMA_E_ARG, "capacities must be > 0");
+    std::vector<float4> cpos, cprev, spos, sprev;
+    std::vector<uint32_t> imask;
+    std::vector<int> xoff{0};
+    std::vector<int2> xent;
+    std::vector<int> comp_off{0}, comp_rep, comp_key;
+    std::vector<float> comp_sphere;
+    std::vector<int> bond_off{0}, angle_off{0}, dih_off{0}, inc_off{0};
+    std::vector<BondD> bonds;
+    std::vector<AngleD> angles;
+    std::vector<DihD> dihs;
+    std::vector<uint32_t> inc;
+    std::vector<int> br_off{0}, br_glob, br_rep, brb_off{0}, bra_off{0}, brd_off{0};
+    std::vector<BondD> brb;
+    std::vector<AngleD> bra;
+    std::vector<DihD> brd;
+    std::vector<int> solv_rep;
+    std::vector<float> mass(nsys * kMaxTypes);
+    std::vector<float2> tsc(nsys * kMaxTypes), tcs(nsys * kMaxTypes), tinter(nsys * 1024), tintra(nsys * 1024), tss(nsys);
+    E->reps_h.clear();
+    E->sys_chain_begin.clear(); E->sys_solv_begin.clear(); E->sys_comp_begin.clear();
+    std::vector<int> bridge_owner;
+
+    for (int r = 0; r < nsys; r++) {
+        const lima_system& s = sys[r];
+        if (!(s.box_len > 0) || !(s.dt > 0)) throw LimaError(LIMA_E_ARG, "box_len and dt must be > 0");
+        if (s.n_compounds < 0 || s.n_solvents < 0 || s.n_bridges < 0) throw LimaError(LIMA_E_ARG, "negative counts");
+        ReplicaInfo ri{};
+        ri.L = s.box_len;
+        ri.half = s.box_len * 0.5f;
+        ri.dt = s.dt;
+        ri.tau = P.thermostat_tau > 0 ? P.thermostat_tau : 1.0;
+        ri.T0 = s.thermostat_target;
+        const int cb = (int)comp_rep.size();
+        const int chb = (int)cpos.size();
+        const int sb = (int)spos.size();
+        E->sys_comp_begin.push_back(cb);
+        E->sys_chain_begin.push_back(chb);
+        E->sys_solv_begin.push_back(sb);
+        ri.comp_begin = cb; ri.comp_end = cb + s.n_compounds;
+        const int nchain = s.n_compounds > 0 ? s.compound_offset[s.n_compounds] : 0;
+        ri.chain_begin = chb; ri.chain_end = chb + nchain;
+        ri.solv_begin = sb; ri.solv_end = sb + s.n_solvents;
+        // type tables (mixing per call site, Engine.cu:643-646, 668-670, 689-691, 711-713, 724-727)
+        const lima_particle_params* pp = s.forcefield.particle_parameters;
+        for (int t = 0; t < kMaxTypes; t++) {
+            mass[r * kMaxTypes + t] = pp[t].mass;
+            float sg = (pp[0].sigma + pp[t].sigma) * 0.5f;
+            float ep = (pp[0].epsilon + pp[t].epsilon) * 0.5f;
+            tsc[r * kMaxTypes + t] = make_float2(sg * sg, 24.f * ep);
+            float epg = sqrtf(pp[t].epsilon * pp[0].epsilon);
+            tcs[r * kMaxTypes + t] = make_float2(sg * sg, 24.f * epg);
+            for (int u = 0; u < kMaxTypes; u++) {
+                float su = (pp[t].sigma + pp[u].sigma) * 0.5f;
+                float ea = (pp[t].epsilon + pp[u].epsilon) * 0.5f;
+                float eg = sqrtf(pp[t].epsilon * pp[u].epsilon);
+                tinter[r * 1024 + t * 32 + u] = make_float2(su * su, 24.f * ea);
+                tintra[r * 1024 + t * 32 + u] = make_float2(su * su, 24.f * eg);
+            }
+        }
+        tss[r] = make_float2(pp[0].sigma * pp[0].sigma, 24.f * pp[0].epsilon);
+        // compounds
+        for (int c = 0; c < s.n_compounds; c++) {
+            const int o0 = s.compound_offset[c], o1 = s.compound_offset[c + 1];
+            const int n = o1 - o0;
+            if (n <= 0 || n > kCompoundBlock) throw LimaError(LIMA_E_CAPACITY, "compound size must be 1..128 particles");
+            if (s.key_particle_index[c] < 0 || s.key_particle_index[c] >= n) throw LimaError(LIMA_E_ARG, "key_particle_index out of range");
+            comp_off.push_back(comp_off.back() + n);
+            comp_rep.push_back(r);
+            comp_key.push_back(s.key_particle_index[c]);
+            comp_sphere.push_back(s.confining_particle_sphere[c]);
+            const int gbase = chb + o0;
+            const int nbd = s.bond_offset[c + 1] - s.bond_offset[c];
+            const int nan = s.angle_offset[c + 1] - s.angle_offset[c];
+            const int ndh = s.dihedral_offset[c + 1] - s.dihedral_offset[c];
+            if (nbd > kMaxBondsPerCompound || nan > kMaxAnglesPerCompound || ndh > kMaxDihedralsPerCompound)
+                throw LimaError(LIMA_E_CAPACITY, "too many bonded terms in a compound");
+            std::vector<std::vector<uint32_t>> pinc(n);
+            for (int t = 0; t < nbd; t++) {
+                const lima_pair_bond& b = s.bonds[s.bond_offset[c] + t];
+                for (int k = 0; k < 2; k++) if ((int)b.atom_indexes[k] >= n) throw LimaError(LIMA_E_ARG, "bond index out of range");
+                bonds.push_back(BondD{b.b0, b.kb, (int)b.atom_indexes[0], (int)b.atom_indexes[1]});
+                for (int k = 0; k < 2; k++) pinc[b.atom_indexes[k]].push_back((0u << 30) | ((uint32_t)t << 2) | (uint32_t)k);
+            }
+            for (int t = 0; t < nan; t++) {
+                const lima_angle_bond& b = s.angles[s.angle_offset[c] + t];
+                for (int k = 0; k < 3; k++) if ((int)b.atom_indexes[k] >= n) throw LimaError(LIMA_E_ARG, "angle index out of range");
+                angles.push_back(AngleD{b.theta_0, b.k_theta, (int)b.atom_indexes[0], (int)b.atom_indexes[1], (int)b.atom_indexes[2], 0});
+                for (int k = 0; k < 3; k++) pinc[b.atom_indexes[k]].push_back((1u << 30) | ((uint32_t)t << 2) | (uint32_t)k);
+            }
+            for (int t = 0; t < ndh; t++) {
+                const lima_dihedral_bond& b = s.dihedrals[s.dihedral_offset[c] + t];
+                for (int k = 0; k < 4; k++) if ((int)b.atom_indexes[k] >= n) throw LimaError(LIMA_E_ARG, "dihedral index out of range");
+                dihs.push_back(DihD{b.phi_0, b.k_phi, (int)b.n, (int)b.atom_indexes[0], (int)b.atom_indexes[1],
+                                    (int)b.atom_indexes[2], (int)b.atom_indexes[3], 0});
+                for (int k = 0; k < 4; k++) pinc[b.atom_indexes[k]].push_back((2u << 30) | ((uint32_t)t << 2) | (uint32_t)k);
+            }
+            bond_off.push_back((int)bonds.size());
+            angle_off.push_back((int)angles.size());
+            dih_off.push_back((int)dihs.size());
+            for (int p = 0; p < n; p++) {
+                for (uint32_t v : pinc[p]) inc.push_back(v);
+                inc_off.push_back((int)inc.size());
+            }
+            for (int p = 0; p < n; p++) {
+                const int li = o0 + p;
+                const float* x = s.chain_positions + 3 * li;
+                const float* xm = s.chain_prev_positions + 3 * li;
+                const int t = s.chain_atom_types[li];
+                if (t >= kMaxTypes) throw LimaError(LIMA_E_ARG, "atom type >= 32");
+                float w;
+                std::memcpy(&w, &t, 4);
+                cpos.push_back(make_float4(x[0], x[1], x[2], w));
+                cprev.push_back(make_float4(xm[0], xm[1], xm[2], 0.f));
+                uint32_t m[4] = {0, 0, 0, 0};
+                for (int e = s.ignore_offset[li]; e < s.ignore_offset[li + 1]; e++) {
+                    const int oc = s.ignore_compound[e], ol = s.ignore_local[e];
+                    if (oc < 0 || oc >= s.n_compounds) throw LimaError(LIMA_E_ARG, "ignore compound id out of range");
+                    if (ol < 0 || ol >= kCompoundBlock) throw LimaError(LIMA_E_ARG, "ignore local id out of range");
+                    if (oc == c) m[ol >> 5] |= 1u << (ol & 31);
+                    else xent.push_back(make_int2(cb + oc, ol));
+                }
+                for (int k = 0; k < 4; k++) imask.push_back(m[k]);
+                xoff.push_back((int)xent.size());
+            }
+            (void)gbase;
+        }
+        // bridges
+        for (int b = 0; b < s.n_bridges; b++) {
+            const int q0 = s.bridge_particle_offset[b], q1 = s.bridge_particle_offset[b + 1];
+            if (q1 - q0 > 32 || q1 - q0 <= 0) throw LimaError(LIMA_E_CAPACITY, "bridge must have 1..32 particles");
+            for (int k = q0; k < q1; k++) {
+                const int cc = s.bridge_particle_compound[k], ll = s.bridge_particle_local[k];
+                if (cc < 0 || cc >= s.n_compounds || ll < 0 || ll >= s.compound_offset[cc + 1] - s.compound_offset[cc])
+                    throw LimaError(LIMA_E_ARG, "bridge particle reference out of range");
+                br_glob.push_back(chb + s.compound_offset[cc] + ll);
+            }
+            br_off.push_back((int)br_glob.size());
+            br_rep.push_back(r);
+            const int np = q1 - q0;
+            for (int t = s.bridge_bond_offset[b]; t < s.bridge_bond_offset[b + 1]; t++) {
+                const lima_pair_bond& x = s.bridge_bonds[t];
+                for (int k = 0; k < 2; k++) if ((int)x.atom_indexes[k] >= np) throw LimaError(LIMA_E_ARG, "bridge bond index out of range");
+                brb.push_back(BondD{x.b0, x.kb, (int)x.atom_indexes[0], (int)x.atom_indexes[1]});
+            }
+            for (int t = s.bridge_angle_offset[b]; t < s.bridge_angle_offset[b + 1]; t++) {
+                const lima_angle_bond& x = s.bridge_angles[t];
+                for (int k = 0; k < 3; k++) if ((int)x.atom_indexes[k] >= np) throw LimaError(LIMA_E_ARG, "bridge angle index out of range");
+                bra.push_back(AngleD{x.theta_0, x.k_theta, (int)x.atom_indexes[0], (int)x.atom_indexes[1], (int)x.atom_indexes[2], 0});
+            }
+            for (int t = s.bridge_dihedral_offset[b]; t < s.bridge_dihedral_offset[b + 1]; t++) {
+                const lima_dihedral_bond& x = s.bridge_dihedrals[t];
+                for (int k = 0; k < 4; k++) if ((int)x.atom_indexes[k] >= np) throw LimaError(LIMA_E_ARG, "bridge dihedral index out of range");
+                brd.push_back(DihD{x.phi_0, x.k_phi, (int)x.n, (int)x.atom_indexes[0], (int)x.atom_indexes[1],
+                                   (int)x.atom_indexes[2], (int)x.atom_indexes[3], 0});
+            }
+            brb_off.push_back((int)brb.size());
+            bra_off.push_back((int)bra.size());
+            brd_off.push_back((int)brd.size());
+        }
+        for (int k = 0; k < s.n_solvents; k++) {
+            const float* x = s.solvent_positions + 3 * k;
+            const float* xm = s.solvent_prev_positions + 3 * k;
+            spos.push_back(make_float4(x[0], x[1], x[2], 0.f));
+            sprev.push_back(make_float4(xm[0], xm[1], xm[2], 0.f));
+            solv_rep.push_back(r);
+        }
+        E->reps_h.push_back(ri);
+    }
+    // a chain particle may belong to at most one bridge (the bridge force is a store)
+    {
+        std::vector<char> seen(cpos.size(), 0);
+        for (int g : br_glob) {
+            if (seen[g]) throw LimaError(LIMA_E_ARG, "a particle belongs to more than one bridge");
+            seen[g] = 1;
+        }
+    }
+    E->R = nsys;
+    E->NC = (int)comp_rep.size();
+    E->NCH = (int)cpos.size();
+    E->NS = (int)spos.size();
+    E->NB = (int)br_rep.size();
+
+    // ---- solvent<-chain work list, solvent tiles, cell grids ----
+    const int spt = E->cfg.spt;
+    const int tile_sz = 256 * spt;
+    E->scwork_h.clear();
+    std::vector<int4> stl;
+    E->kmax = 0;
+    int cell_begin = 0;
+    for (int r = 0; r < nsys; r++) {
+        ReplicaInfo& ri = E->reps_h[r];
+        // compound chunks: consecutive compounds up to the target particle count
+        std::vector<std::pair<int, int>> chunks;
+        int c = ri.comp_begin;
+        while (c < ri.comp_end) {
+            int c1 = c, np = 0;
+            while (c1 < ri.comp_end && (c1 - c) < kChunkCompounds) {
+                int n = comp_off[c1 + 1] - comp_off[c1];
+                if (np > 0 && (np + n > E->cfg.chunk_particles || np + n > kChunkParticles)) break;
+                np += n;
+                c1++;
+            }
+            chunks.push_back({c, c1});
+            c = c1;
+        }
+        const int ns_r = ri.solv_end - ri.solv_begin;
+        if (ns_r == 0) chunks.clear();
+        ri.chunk_count = (int)chunks.size();
+        E->kmax = std::max(E->kmax, ri.chunk_count);
+        for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += tile_sz)
+            for (int k = 0; k < (int)chunks.size(); k++)
+                E->scwork_h.push_back(SCWork{r, t0, std::min(t0 + tile_sz, ri.solv_end), chunks[k].first, chunks[k].second, k, 0, 0});
+        ri.stile_begin = (int)stl.size();
+        for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += 256)
+            stl.push_back(make_int4(r, t0, std::min(t0 + 256, ri.solv_end), (int)stl.size()));
+        ri.stile_end = (int)stl.size();
+        const float thr = P.cutoff + P.cutoff;
+        int ncell = (int)std::floor(ri.L / (thr * (1.0f + 1e-5f)));
+        ncell = std::max(1, std::min(ncell, 256));
+        ri.ncell = ncell;
+        ri.cell_inv = (float)ncell / ri.L;
+        ri.cell_begin = cell_begin;
+        cell_begin += ncell * ncell * ncell;
+    }
+    // heaviest work items first
+    std::stable_sort(E->scwork_h.begin(), E->scwork_h.end(), [&](const SCWork& x, const SCWork& y) {
+        long long wx = (long long)(x.solv_end - x.solv_begin) * (comp_off[x.comp_end] - comp_off[x.comp_begin]);
+        long long wy = (long long)(y.solv_end - y.solv_begin) * (comp_off[y.comp_end] - comp_off[y.comp_begin]);
+        return wx > wy;
+    });
+    E->n_cells = std::max(cell_begin, 1);
+    E->n_stiles = (int)stl.size();
+
+    // ---- upload ----
+    E->reps.upload(E->reps_h);
+    for (int k = 0; k < 2; k++) E->cpos[k].upload(cpos);
+    E->cprev.upload(cprev);
+    E->cforce.alloc(std::max(E->NCH, 1)); E->cforce.zero();
+    E->fbridge.alloc(std::max(E->NCH, 1)); E->fbridge.zero();
+    E->intra_mask.upload(imask);
+    E->xoff.upload(xoff);
+    E->xent.upload(xent);
+    E->comp_off.upload(comp_off);
+    E->comp_rep.upload(comp_rep);
+    E->comp_key.upload(comp_key);
+    E->comp_sphere.upload(comp_sphere);
+    E->bond_off.upload(bond_off); E->bonds.upload(bonds);
+    E->angle_off.upload(angle_off); E->angles.upload(angles);
+    E->dih_off.upload(dih_off); E->dihs.upload(dihs);
+    E->inc_off.upload(inc_off); E->inc.upload(inc);
+    E->br_off.upload(br_off); E->br_glob.upload(br_glob); E->br_rep.upload(br_rep);
+    E->brb_off.upload(brb_off); E->brb.upload(brb);
+    E->bra_off.upload(bra_off); E->bra.upload(bra);
+    E->brd_off.upload(brd_off); E->brd.upload(brd);
+    for (int k = 0; k < 2; k++) E->spos[k].upload(spos);
+    E->sprev.upload(sprev);
+    E->sforce.alloc(std::max(E->NS, 1)); E->sforce.zero();
+    E->solv_rep.upload(solv_rep);
+    E->stiles.upload(stl);
+    E->mass.upload(mass);
+    E->tab_sc.upload(tsc); E->tab_cs.upload(tcs); E->tab_inter.upload(tinter); E->tab_intra.upload(tintra); E->tab_ss.upload(tss);
+    const size_t NCx = std::max(E->NC, 1), NSx = std::max(E->NS, 1);
+    E->cc_cnt.alloc(NCx); E->cc_cnt.zero();
+    E->cc_old_cnt.alloc(NCx); E->cc_old_cnt.zero();
+    E->cc_ids.alloc(NCx * P.max_compound_neighbors);
+    E->cc_old_ids.alloc(NCx * P.max_compound_neighbors);
+    E->cs_cnt.alloc(NCx); E->cs_cnt.zero();
+    E->cs_ids.alloc(NCx * P.max_compound_solvents);
+    E->ss_cnt.alloc(NSx); E->ss_cnt.zero();
+    E->ss_ids.alloc(NSx * P.max_solvent_neighbors);
+    E->cw_off.alloc(NCx + 1); E->cw_off.zero();
+    E->cw_total.alloc(1); E->cw_total.zero();
+    E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
+    if (P.accumulation == LIMA_ACCUM_FIXED64) {
+        E->partC_fx.alloc(E->max_slots * kCompoundBlock);
+        E->partS_fx.alloc(std::max<size_t>(1, (size_t)E->kmax * NSx));
+        E->partC.alloc(1); E->partS.alloc(1);
+    } else {
+        E->partC.alloc(E->max_slots * kCompoundBlock);
+        E->partS.alloc(std::max<size_t>(1, (size_t)E->kmax * NSx));
+        E->partC_fx.alloc(1); E->partS_fx.alloc(1);
+    }
+    E->scwork.upload(E->scwork_h);
+    std::vector<float> ones(nsys, 1.0f);
+    E->lambda.upload(ones);
+    E->lambda_applied.upload(ones);
+    E->ke_comp.alloc(NCx); E->ke_comp.zero();
+    E->ke_stile.alloc(std::max(E->n_stiles, 1)); E->ke_stile.zero();
+    E->temperature.alloc(nsys); E->temperature.zero();
+    E->energy.alloc((size_t)nsys * kEnergyTerms); E->energy.zero();
+    E->err.alloc(4); E->err.zero();
+    E->cell_cnt.alloc(E->n_cells); E->cell_start.alloc(E->n_cells + 1); E->cell_cursor.alloc(E->n_cells);
+    E->cell_ids.alloc(NSx); E->solv_cell.alloc(NSx);
+}
+
+thread_local std::string g_last_error;
+
+int fail(lima_engine* E, const LimaError& e) {
+    if (E) E->last_error = e.msg;
+    g_last_error = e.msg;
+    return e.code;
+}
+
+template <typename F> int guarded(lima_engine* E, F&& f) {
+    if (!E) return LIMA_E_ARG;
+    try {
+        return f();
+    } catch (const LimaError& e) {
+        return fail(E, e);
+    }
+}
+
+void get3(const std::vector<float4>& v, float* out) {
+    for (size_t i = 0; i < v.size(); i++) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lima_version(void) { return "lima-mi355x 0.1.0 (gfx950)"; }
+
+int lima_device_count(int* count) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return LIMA_OK;
+}
+
+int lima_create(const lima_system* systems, int32_t n_systems, const lima_run_params* params, lima_engine** out) {
+    if (!out || !systems || !params) return LIMA_E_ARG;
+    *out = nullptr;
+    lima_engine* E = new lima_engine();
+    E->params = *params;
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaError(LIMA_E_HIP, "no HIP device available");
+        if (params->device < 0 || params->device >= ndev) throw LimaError(LIMA_E_ARG, "device ordinal out of range");
+        E->device = params->device;
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+        validate_and_build(E, systems, n_systems);
+        // Engine::Engine -> handleNLISTS(force) (Engine.cu:35)
+        E->launch_rebuild(E->cur);
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        int st = E->check_errors();
+        if (st != LIMA_OK) {
+            g_last_error = E->last_error;
+            delete E;
+            return st;
+        }
+    } catch (const LimaError& e) {
+        g_last_error = e.msg;
+        delete E;
+        return e.code;
+    }
+    *out = E;
+    return LIMA_OK;
+}
+
+void lima_destroy(lima_engine* E) {
+    if (!E) return;
+    (void)hipSetDevice(E->device);
+    (void)hipStreamSynchronize(E->stream);
+    delete E;
+}
+
+const char* lima_last_error(const lima_engine* E) { return E ? E->last_error.c_str() : g_last_error.c_str(); }
+
+int lima_step_async(lima_engine* E, int32_t n) {
+    return guarded(E, [&] {
+        if (n < 0) throw LimaError(LIMA_E_ARG, "n_steps < 0");
+        HIP_TRY(hipSetDevice(E->device));
+        E->enqueue_steps(n);
+        return LIMA_OK;
+    });
+}
+
+int lima_sync(lima_engine* E) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        return E->check_errors();
+    });
+}
+
+int lima_step(lima_engine* E, int32_t n) {
+    return guarded(E, [&] {
+        if (n < 0) throw LimaError(LIMA_E_ARG, "n_steps < 0");
+        HIP_TRY(hipSetDevice(E->device));
+        auto t0 = std::chrono::steady_clock::now();
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, E->stream));
+        E->enqueue_steps(n);
+        HIP_TRY(hipEventRecord(e1, E->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        E->t_device_us += ms * 1000.0;
+        E->t_host_us += std::chrono::duration<double, std::micro>(t1 - t0).count();
+        return E->check_errors();
+    });
+}
+
+int lima_rebuild_neighbors(lima_engine* E) {
+    return guarded(E, [&] {
+        HIP_TRY(hipSetDevice(E->device));
+        E->launch_rebuild(E->cur);
+        E->since_rebuild = 0;
+        E->n_rebuilds++;
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        return E->check_errors();
+    });
+}
+
+int lima_get_state(lima_engine* E, float* cx, float* cxm, float* sx, float* sxm) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        std::vector<float4> h;
+        auto pull = [&](DBuf<float4>& b, int n, float* out) {
+            if (!out || n == 0) return;
+            h.resize(n);
+            HIP_TRY(hipMemcpy(h.data(), b.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+            get3(h, out);
+        };
+        pull(E->cpos[E->cur], E->NCH, cx);
+        pull(E->cprev, E->NCH, cxm);
+        pull(E->spos[E->cur], E->NS, sx);
+        pull(E->sprev, E->NS, sxm);
+        return LIMA_OK;
+    });
+}
+
+int lima_set_state(lima_engine* E, const float* cx, const float* cxm, const float* sx, const float* sxm) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        std::vector<float4> h;
+        auto push = [&](DBuf<float4>& b, int n, const float* in, bool keep_w) {
+            if (!in || n == 0) return;
+            h.resize(n);
+            HIP_TRY(hipMemcpy(h.data(), b.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) h[i] = make_float4(in[3 * i], in[3 * i + 1], in[3 * i + 2], keep_w ? h[i].w : 0.f);
+            HIP_TRY(hipMemcpy(b.p, h.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        };
+        push(E->cpos[E->cur], E->NCH, cx, true);
+        push(E->cprev, E->NCH, cxm, false);
+        push(E->spos[E->cur], E->NS, sx, false);
+        push(E->sprev, E->NS, sxm, false);
+        return LIMA_OK;
+    });
+}
+
+int lima_get_forces(lima_engine* E, float* cf, float* sf) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        std::vector<float4> h;
+        if (cf && E->NCH) { h.resize(E->NCH); HIP_TRY(hipMemcpy(h.data(), E->cforce.p, E->NCH * sizeof(float4), hipMemcpyDeviceToHost)); get3(h, cf); }
+        if (sf && E->NS) { h.resize(E->NS); HIP_TRY(hipMemcpy(h.data(), E->sforce.p, E->NS * sizeof(float4), hipMemcpyDeviceToHost)); get3(h, sf); }
+        return LIMA_OK;
+    });
+}
+
+int lima_compute_forces(lima_engine* E, float* cf, float* sf) {
+    return guarded(E, [&] {
+        HIP_TRY(hipSetDevice(E->device));
+        if (E->params.accumulation == LIMA_ACCUM_FIXED64) E->launch_forces<false, true, false>(E->cur);
+        else E->launch_forces<false, false, false>(E->cur);
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        int st = E->check_errors();
+        if (st != LIMA_OK) return st;
+        return lima_get_forces(E, cf, sf);
+    });
+}
+
+int lima_get_energies(lima_engine* E, lima_energies* out) {
+    return guarded(E, [&] {
+        if (!out) throw LimaError(LIMA_E_ARG, "null output");
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
+        if (E->params.accumulation == LIMA_ACCUM_FIXED64) E->launch_forces<false, true, true>(E->cur);
+        else E->launch_forces<false, false, true>(E->cur);
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        std::vector<double> e(E->energy.n);
+        HIP_TRY(hipMemcpy(e.data(), E->energy.p, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<float> lam(E->R);
+        HIP_TRY(hipMemcpy(lam.data(), E->lambda_applied.p, E->R * sizeof(float), hipMemcpyDeviceToHost));
+        for (int r = 0; r < E->R; r++) {
+            const double* v = &e[(size_t)r * kEnergyTerms];
+            const ReplicaInfo& ri = E->reps_h[r];
+            lima_energies& o = out[r];
+            std::memset(&o, 0, sizeof(o));
+            o.bond = v[kE_bond]; o.angle = v[kE_angle]; o.dihedral = v[kE_dih];
+            o.lj_intra = 0.5 * v[kE_intra]; o.lj_inter = 0.5 * v[kE_inter];
+            o.lj_compound_solvent = 0.5 * v[kE_cs]; o.lj_solvent_compound = 0.5 * v[kE_sc];
+            o.lj_solvent_solvent = 0.5 * v[kE_ss];
+            o.potential = o.bond + o.angle + o.dihedral + o.lj_intra + o.lj_inter + o.lj_compound_solvent +
+                          o.lj_solvent_compound + o.lj_solvent_solvent;
+            o.kinetic = 0.5 * v[kE_kin] / (ri.dt * ri.dt);
+            const double ntot = (double)((ri.chain_end - ri.chain_begin) + (ri.solv_end - ri.solv_begin));
+            o.temperature = ntot > 0 ? v[kE_kin] / (ri.dt * ri.dt) / (3.0 * ntot * 8.3145) : 0.0;
+            o.virial = 0.5 * v[kE_virial];
+            o.thermostat_scalar = lam[r];
+        }
+        return E->check_errors();
+    });
+}
+
+int lima_get_neighbor_counts(lima_engine* E, int32_t* ccc, int32_t* csc, int32_t* ssc) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        if (ccc && E->NC) HIP_TRY(hipMemcpy(ccc, E->cc_cnt.p, E->NC * sizeof(int), hipMemcpyDeviceToHost));
+        if (csc && E->NC) HIP_TRY(hipMemcpy(csc, E->cs_cnt.p, E->NC * sizeof(int), hipMemcpyDeviceToHost));
+        if (ssc && E->NS) HIP_TRY(hipMemcpy(ssc, E->ss_cnt.p, E->NS * sizeof(int), hipMemcpyDeviceToHost));
+        return LIMA_OK;
+    });
+}
+
+int lima_get_neighbors(lima_engine* E, int32_t* cci, int32_t* csi, int32_t* ssi) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        const lima_run_params& P = E->params;
+        if (cci && E->NC) HIP_TRY(hipMemcpy(cci, E->cc_ids.p, (size_t)E->NC * P.max_compound_neighbors * sizeof(int), hipMemcpyDeviceToHost));
+        if (csi && E->NC) HIP_TRY(hipMemcpy(csi, E->cs_ids.p, (size_t)E->NC * P.max_compound_solvents * sizeof(int), hipMemcpyDeviceToHost));
+        if (ssi && E->NS) {
+            std::vector<int> t((size_t)E->NS * P.max_solvent_neighbors);
+            HIP_TRY(hipMemcpy(t.data(), E->ss_ids.p, t.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (int s = 0; s < E->NS; s++)
+                for (int k = 0; k < P.max_solvent_neighbors; k++) ssi[(size_t)s * P.max_solvent_neighbors + k] = t[(size_t)k * E->NS + s];
+        }
+        return LIMA_OK;
+    });
+}
+
+int lima_set_neighbors(lima_engine* E, const int32_t* ccc, const int32_t* cci, const int32_t* csc, const int32_t* csi,
+                       const int32_t* ssc, const int32_t* ssi) {
+    return guarded(E, [&] {
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        const lima_run_params& P = E->params;
+        if (ccc && cci && E->NC) {
+            for (int c = 0; c < E->NC; c++) if (ccc[c] < 0 || ccc[c] > P.max_compound_neighbors) throw LimaError(LIMA_E_CAPACITY, "compound list count exceeds capacity");
+            HIP_TRY(hipMemcpy(E->cc_cnt.p, ccc, E->NC * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E->cc_ids.p, cci, (size_t)E->NC * P.max_compound_neighbors * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (csc && csi && E->NC) {
+            for (int c = 0; c < E->NC; c++) if (csc[c] < 0 || csc[c] > P.max_compound_solvents) throw LimaError(LIMA_E_CAPACITY, "compound solvent list count exceeds capacity");
+            HIP_TRY(hipMemcpy(E->cs_cnt.p, csc, E->NC * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E->cs_ids.p, csi, (size_t)E->NC * P.max_compound_solvents * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (ssc && ssi && E->NS) {
+            for (int s = 0; s < E->NS; s++) if (ssc[s] < 0 || ssc[s] > P.max_solvent_neighbors) throw LimaError(LIMA_E_CAPACITY, "solvent list count exceeds capacity");
+            std::vector<int> t((size_t)E->NS * P.max_solvent_neighbors, 0);
+            for (int s = 0; s < E->NS; s++)
+                for (int k = 0; k < ssc[s]; k++) t[(size_t)k * E->NS + s] = ssi[(size_t)s * P.max_solvent_neighbors + k];
+            HIP_TRY(hipMemcpy(E->ss_cnt.p, ssc, E->NS * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E->ss_ids.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (E->NC) {
+            NlistArgs a = E->nlist_args(E->cur);
+            nl_work_kernel<<<1, 1024, 0, E->stream>>>(a);
+            HIP_TRY(hipStreamSynchronize(E->stream));
+        }
+        return LIMA_OK;
+    });
+}
+
+int lima_get_timings(lima_engine* E, double* out5) {
+    return guarded(E, [&] {
+        if (!out5) throw LimaError(LIMA_E_ARG, "null output");
+        out5[0] = E->t_device_us;
+        out5[1] = E->t_rebuild_us;
+        out5[2] = E->t_host_us;
+        out5[3] = (double)E->step;
+        out5[4] = (double)E->n_rebuilds;
+        return LIMA_OK;
+    });
+}
+
+int lima_time_kernels(lima_engine* E, int32_t reps, double* out6) {
+    return guarded(E, [&] {
+        if (!out6 || reps <= 0) throw LimaError(LIMA_E_ARG, "bad arguments");
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        StepArgs a = E->step_args(E->cur);
+        const bool fx = E->params.accumulation == LIMA_ACCUM_FIXED64;
+        auto timeit = [&](auto&& launch) -> double {
+            launch();   // warm
+            HIP_TRY(hipEventRecord(e0, E->stream));
+            for (int i = 0; i < reps; i++) launch();
+            HIP_TRY(hipEventRecord(e1, E->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            return ms * 1000.0 / reps;
+        };
+        for (int k = 0; k < 6; k++) out6[k] = 0.0;
+        const int nw = (int)E->scwork_h.size();
+        if (E->params.enable_solvents && nw > 0) {
+            out6[0] = timeit([&] {
+                if (fx) {
+                    if (E->cfg.spt == 1) solvent_chain_kernel<1, true, false><<<nw, 256, 0, E->stream>>>(a);
+                    else if (E->cfg.spt == 4) solvent_chain_kernel<4, true, false><<<nw, 256, 0, E->stream>>>(a);
+                    else solvent_chain_kernel<2, true, false><<<nw, 256, 0, E->stream>>>(a);
+                } else {
+                    if (E->cfg.spt == 1) solvent_chain_kernel<1, false, false><<<nw, 256, 0, E->stream>>>(a);
+                    else if (E->cfg.spt == 4) solvent_chain_kernel<4, false, false><<<nw, 256, 0, E->stream>>>(a);
+                    else solvent_chain_kernel<2, false, false><<<nw, 256, 0, E->stream>>>(a);
+                }
+            });
+        }
+        if (E->NC > 0) {
+            const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
+            out6[1] = timeit([&] {
+                if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+            });
+            out6[2] = timeit([&] {
+                if (fx) compound_integrate_kernel<false, true, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
+                else compound_integrate_kernel<false, false, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
+            });
+        }
+        if (E->params.enable_solvents && E->NS > 0) {
+            out6[3] = timeit([&] {
+                if (fx) solvent_integrate_kernel<false, true, false><<<E->n_stiles, 256, 0, E->stream>>>(a);
+                else solvent_integrate_kernel<false, false, false><<<E->n_stiles, 256, 0, E->stream>>>(a);
+            });
+        }
+        if (E->NB > 0) out6[4] = timeit([&] { bridge_kernel<false><<<(E->NB + 63) / 64, 64, 0, E->stream>>>(a); });
+        out6[5] = timeit([&] { E->launch_rebuild(E->cur); });
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return E->check_errors();
+    });
+}
+
+int lima_get_config(lima_engine* E, int32_t* out8) {
+    return guarded(E, [&] {
+        if (!out8) throw LimaError(LIMA_E_ARG, "null output");
+        out8[0] = E->cfg.spt;
+        out8[1] = E->cfg.chunk_particles;
+        out8[2] = (int)E->scwork_h.size();
+        out8[3] = E->kmax;
+        out8[4] = E->cfg.use_graph;
+        out8[5] = E->n_cells;
+        out8[6] = E->n_stiles;
+        out8[7] = E->R;
+        return LIMA_OK;
+    });
+}
+
+int lima_autotune(lima_engine* E, int32_t trial_steps) {
+    (void)trial_steps;
+    return guarded(E, [&] { return (int)LIMA_E_UNSUPPORTED; });
+}
+
+}  // extern "C"

@@ -1,0 +1,915 @@
+// kernels.h — gfx950 kernels of one LIMA_MD step and of the neighbour-list build.
+//
+// Mapping to the reference (Quantom/Quantom/Engine.cu):
+//   bridge_kernel              compoundBridgeKernel          1168-1223
+//   solvent_chain_kernel       solventForceKernel 1099-1117 (all-pairs solvent <- every chain particle)
+//   compound_listed_kernel     compoundKernel 939-974 (inter-compound and solvent-list LJ)
+//   compound_integrate_kernel  compoundKernel 918-931, 981-1056 (bonded, intra-LJ, integrate, PBC)
+//   solvent_integrate_kernel   solventForceKernel 1121-1155 (solvent-solvent LJ, integrate, PBC)
+//   thermostat_kernel          Engine::handleBoxtemp 362-393, as an FP64 device reduction
+//   nl_*                       Engine::updateNeighborLists 146-287, on the device
+//
+// Every kernel is replica-aware: entities carry their replica id and read the
+// box length, time step and type tables of that replica, so one launch covers
+// any number of independent systems.
+#pragma once
+#include "bonded.h"
+#include "device_common.h"
+
+namespace lima {
+
+struct I3 {
+    long long x, y, z;
+};
+
+struct StepArgs {
+    const ReplicaInfo* reps;
+    int n_replicas;
+    // chain particles
+    const float4* cpos;      // current (x, y, z, type bits)
+    float4* cpos_next;
+    float4* cprev;
+    float4* cforce;
+    float4* fbridge;
+    const uint32_t* intra_mask;   // [n_chain*4]
+    const int* xoff;              // cross-compound exclusions [n_chain+1]
+    const int2* xent;             // (compound, local)
+    // compounds
+    int n_comp;
+    const int* comp_off;          // [n_comp+1]
+    const int* comp_rep;
+    const int* comp_key;          // compound-local key index
+    const float* comp_sphere;
+    const int* bond_off; const BondD* bonds;
+    const int* angle_off; const AngleD* angles;
+    const int* dih_off; const DihD* dihs;
+    const int* inc_off; const uint32_t* inc;   // per chain particle: term incidences
+    // bridges
+    int n_bridges;
+    const int* br_off; const int* br_glob; const int* br_rep;
+    const int* brb_off; const BondD* brb;
+    const int* bra_off; const AngleD* bra;
+    const int* brd_off; const DihD* brd;
+    // solvents
+    int n_solv;
+    const float4* spos;
+    float4* spos_next;
+    float4* sprev;
+    float4* sforce;
+    const int* solv_rep;
+    const int4* stiles;           // (replica, begin, end, tile index) per solvent-integrate tile
+    // type tables per replica
+    const float* mass;            // [R*32]
+    const float2* tab_sc;         // [R*32] solvent <- chain type (sig2, 24 eps), arithmetic eps
+    const float2* tab_cs;         // [R*32] chain type <- solvent, geometric eps
+    const float2* tab_inter;      // [R*1024] arithmetic
+    const float2* tab_intra;      // [R*1024] geometric
+    const float2* tab_ss;         // [R]
+    // lists
+    int cap_cc, cap_cs, cap_ss;
+    const int* cc_cnt; const int* cc_ids;
+    const int* cs_cnt; const int* cs_ids;
+    const int* ss_cnt; const int* ss_ids;   // transposed: ss_ids[k*n_solv + s]
+    const int* cw_off; const int* cw_total;
+    // partial forces
+    float4* partC;  I3* partC_fx;   // [slot*128 + p]
+    float4* partS;  I3* partS_fx;   // [chunk*n_solv + s]
+    const SCWork* scwork; int n_scwork;
+    // thermostat
+    const float* lambda;          // [R] applied this step
+    double* ke_comp;              // [n_comp]
+    double* ke_stile;             // [n_stiles]
+    // energies: [R*kEnergyTerms]
+    double* energy;
+    uint32_t* err;                // [4]
+    int enable_solvents, enable_dihedrals;
+};
+
+enum { kE_bond = 0, kE_angle, kE_dih, kE_intra, kE_inter, kE_cs, kE_sc, kE_ss, kE_virial, kE_kin, kEnergyTerms };
+
+constexpr int kChunkParticles = 2048;   // chain particles staged per solvent<-chain work item
+constexpr int kChunkCompounds = 64;
+
+LIMA_DEV int ftype(float4 p) { return __float_as_int(p.w); }
+
+LIMA_DEV void block_atomic_energy(double v, double* dst) {
+    __shared__ double red[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) s += red[w];
+        if (s != 0.0) atomicAdd(dst, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compoundBridgeKernel (Engine.cu:1168-1223): one thread per bridge, terms in
+// index order, forces stored (not added) into the carry-in buffer.
+// ---------------------------------------------------------------------------
+template <bool ENERGY>
+__global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.n_bridges) return;
+    const ReplicaInfo rp = a.reps[a.br_rep[b]];
+    const int p0 = a.br_off[b], np = a.br_off[b + 1] - p0;
+    constexpr int MAXP = 32;
+    F3 pos[MAXP], ps[MAXP], as[MAXP], ds[MAXP];
+    for (int k = 0; k < np && k < MAXP; k++) {
+        float4 q = a.cpos[a.br_glob[p0 + k]];
+        pos[k] = f3(q.x, q.y, q.z);
+        ps[k] = as[k] = ds[k] = f3(0.f, 0.f, 0.f);
+    }
+    for (int k = 0; k < np && k < MAXP; k++) hyperpos(pos[0].x, pos[0].y, pos[0].z, pos[k].x, pos[k].y, pos[k].z, rp.L, rp.half);
+    double eb = 0, ea = 0, ed = 0;
+    for (int t = a.brb_off[b]; t < a.brb_off[b + 1]; t++) {
+        BondD bd = a.brb[t];
+        F3 r[2];
+        pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, &eb);
+        ps[bd.a0] = f3_add(ps[bd.a0], r[0]);
+        ps[bd.a1] = f3_add(ps[bd.a1], r[1]);
+    }
+    for (int t = a.bra_off[b]; t < a.bra_off[b + 1]; t++) {
+        AngleD ad = a.bra[t];
+        F3 r[3];
+        anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, &ea);
+        as[ad.a0] = f3_add(as[ad.a0], r[0]);
+        as[ad.a1] = f3_add(as[ad.a1], r[1]);
+        as[ad.a2] = f3_add(as[ad.a2], r[2]);
+    }
+    if (a.enable_dihedrals) {
+        for (int t = a.brd_off[b]; t < a.brd_off[b + 1]; t++) {
+            DihD dd = a.brd[t];
+            F3 r[4];
+            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, &ed);
+            ds[dd.a0] = f3_add(ds[dd.a0], r[0]);
+            ds[dd.a1] = f3_add(ds[dd.a1], r[1]);
+            ds[dd.a2] = f3_add(ds[dd.a2], r[2]);
+            ds[dd.a3] = f3_add(ds[dd.a3], r[3]);
+        }
+    }
+    for (int k = 0; k < np && k < MAXP; k++) {
+        F3 f = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps[k]), as[k]), ds[k]);
+        a.fbridge[a.br_glob[p0 + k]] = make_float4(f.x, f.y, f.z, 0.0f);
+    }
+    if (ENERGY) {
+        double* e = a.energy + a.br_rep[b] * kEnergyTerms;
+        atomicAdd(e + kE_bond, eb);
+        atomicAdd(e + kE_angle, ea);
+        atomicAdd(e + kE_dih, ed);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solventForceKernel's compound loop (Engine.cu:1099-1117): every solvent
+// against every chain particle of its replica, no cutoff.  Work item =
+// (solvent tile, compound chunk).  The chunk's particles are staged in LDS as
+// (x, y, z, sigma^2) + 24*eps; every lane holds SPT solvents.  The solvent's own
+// position is re-imaged against each compound's particle 0 in compound order,
+// exactly as the reference mutates it (Engine.cu:1111), by replaying the
+// compounds before the chunk.  Per-compound partial sums match
+// computeLJForces' local sum (Engine.cu:640-651).
+// ---------------------------------------------------------------------------
+template <int SPT, bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
+    __shared__ float4 tile[kChunkParticles];
+    __shared__ float tile_e[kChunkParticles];
+    __shared__ int tile_off[kChunkCompounds + 1];
+    __shared__ float2 tab[kMaxTypes];
+    const SCWork wk = a.scwork[blockIdx.x];
+    const ReplicaInfo rp = a.reps[wk.replica];
+    const float L = rp.L, half = rp.half;
+    const int tid = threadIdx.x;
+    if (tid < kMaxTypes) tab[tid] = a.tab_sc[wk.replica * kMaxTypes + tid];
+    const int c0 = wk.comp_begin, ncomp = wk.comp_end - wk.comp_begin;
+    const int base = a.comp_off[c0], total = a.comp_off[wk.comp_end] - base;
+    for (int k = tid; k <= ncomp; k += blockDim.x) tile_off[k] = a.comp_off[c0 + k] - base;
+    __syncthreads();
+    for (int k = tid; k < total; k += blockDim.x) {
+        float4 p = a.cpos[base + k];
+        float2 tp = tab[ftype(p)];
+        tile[k] = make_float4(p.x, p.y, p.z, tp.x);
+        tile_e[k] = tp.y;
+    }
+    float sx[SPT], sy[SPT], sz[SPT];
+    float fx[SPT], fy[SPT], fz[SPT];
+    long long ix[SPT], iy[SPT], iz[SPT];
+    double en[SPT], vir[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        int s = wk.solv_begin + tid + k * blockDim.x;
+        float4 p = a.spos[s < wk.solv_end ? s : wk.solv_begin];
+        sx[k] = p.x; sy[k] = p.y; sz[k] = p.z;
+        fx[k] = fy[k] = fz[k] = 0.f;
+        ix[k] = iy[k] = iz[k] = 0;
+        en[k] = vir[k] = 0.0;
+    }
+    // replay the reference's sequential re-imaging over the preceding compounds
+    for (int c = rp.comp_begin; c < c0; c++) {
+        float4 q = a.cpos[a.comp_off[c]];
+#pragma unroll
+        for (int k = 0; k < SPT; k++) hyperpos(q.x, q.y, q.z, sx[k], sy[k], sz[k], L, half);
+    }
+    __syncthreads();
+    for (int ci = 0; ci < ncomp; ci++) {
+        const int j0 = tile_off[ci], j1 = tile_off[ci + 1];
+        const float4 q0 = tile[j0];
+#pragma unroll
+        for (int k = 0; k < SPT; k++) hyperpos(q0.x, q0.y, q0.z, sx[k], sy[k], sz[k], L, half);
+        float lx[SPT], ly[SPT], lz[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++) lx[k] = ly[k] = lz[k] = 0.f;
+#pragma unroll 4
+        for (int j = j0; j < j1; j++) {
+            const float4 q = tile[j];
+            const float e = tile_e[j];
+#pragma unroll
+            for (int k = 0; k < SPT; k++) {
+                if (ENERGY) lj_acc_energy(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e, lx[k], ly[k], lz[k], en[k], vir[k]);
+                else lj_acc(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e, lx[k], ly[k], lz[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+            if (FIXED) { ix[k] += to_fixed(lx[k]); iy[k] += to_fixed(ly[k]); iz[k] += to_fixed(lz[k]); }
+            else { fx[k] += lx[k]; fy[k] += ly[k]; fz[k] += lz[k]; }
+        }
+    }
+    double esum = 0, vsum = 0;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        int s = wk.solv_begin + tid + k * blockDim.x;
+        if (s < wk.solv_end) {
+            size_t slot = (size_t)wk.chunk * a.n_solv + s;
+            if (FIXED) a.partS_fx[slot] = I3{ix[k], iy[k], iz[k]};
+            else a.partS[slot] = make_float4(fx[k], fy[k], fz[k], 0.f);
+            esum += en[k];
+            vsum += vir[k];
+        }
+    }
+    if (ENERGY) {
+        block_atomic_energy(esum, a.energy + wk.replica * kEnergyTerms + kE_sc);
+        block_atomic_energy(vsum, a.energy + wk.replica * kEnergyTerms + kE_virial);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compoundKernel's listed LJ (Engine.cu:939-974): work item = (compound,
+// source chunk); sources are one neighbour compound (arithmetic eps, LJ
+// exclusions) or 256 entries of the compound's solvent list (geometric eps).
+// Sources are re-imaged against the compound's particle 0 (Engine.cu:949, 967).
+// One thread per target particle; partial forces go to the compound's slots.
+// ---------------------------------------------------------------------------
+LIMA_DEV int find_compound(const int* cw_off, int n, int w) {
+    int lo = 0, hi = n;   // largest c with cw_off[c] <= w
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (cw_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
+    __shared__ float4 src[kSolventChunk];
+    __shared__ float2 tab[kMaxTypes * kMaxTypes];
+    __shared__ int cached_rep;
+    const int p = threadIdx.x;
+    if (p == 0) cached_rep = -1;
+    const int total = *a.cw_total;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int c = find_compound(a.cw_off, a.n_comp, w);
+        const int k = w - a.cw_off[c];
+        const int r = a.comp_rep[c];
+        const ReplicaInfo rp = a.reps[r];
+        const float L = rp.L, half = rp.half;
+        const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
+        const bool active = p < n;
+        float4 me = a.cpos[base + (active ? p : 0)];
+        const float4 p0 = a.cpos[base];
+        hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);
+        const int tp = ftype(me);
+        const int ncc = a.cc_cnt[c];
+        __syncthreads();
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        double en = 0.0, vir = 0.0;
+        if (k < ncc) {
+            if (cached_rep != r) {
+                for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) tab[t] = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + t];
+            }
+            const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
+            const int nbase = a.comp_off[nb], nn = a.comp_off[nb + 1] - nbase;
+            if (p < nn) {
+                float4 q = a.cpos[nbase + p];
+                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                src[p] = q;
+            }
+            // LJ_Ignores of this particle against the neighbour (Engine.cu:660)
+            uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+            if (active) {
+                for (int e = a.xoff[base + p]; e < a.xoff[base + p + 1]; e++) {
+                    int2 x = a.xent[e];
+                    if (x.x == nb) {
+                        uint32_t bit = 1u << (x.y & 31);
+                        int wsel = x.y >> 5;
+                        m0 |= wsel == 0 ? bit : 0u; m1 |= wsel == 1 ? bit : 0u;
+                        m2 |= wsel == 2 ? bit : 0u; m3 |= wsel == 3 ? bit : 0u;
+                    }
+                }
+            }
+            __syncthreads();
+            if (p == 0) cached_rep = r;
+            if (active) {
+                for (int g = 0; g * 32 < nn; g++) {
+                    const uint32_t mw = g == 0 ? m0 : g == 1 ? m1 : g == 2 ? m2 : m3;
+                    const int jend = min(nn, g * 32 + 32);
+                    for (int j = g * 32; j < jend; j++) {
+                        const float4 q = src[j];
+                        const float2 pr = tab[ftype(q) * kMaxTypes + tp];
+                        float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                        float r2 = dx * dx + dy * dy + dz * dz;
+                        float inv = __builtin_amdgcn_rcpf(r2);
+                        float s = pr.x * inv;
+                        float s3 = s * s * s;
+                        float fs = pr.y * s3 * inv * (1.0f - 2.0f * s3);
+                        const bool ex = (mw >> (j & 31)) & 1u;
+                        fs = ex ? 0.0f : fs;
+                        lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                        if (ENERGY && !ex) {
+                            double sd = (double)pr.x / (double)r2, s6 = sd * sd * sd;
+                            en += (double)pr.y / 6.0 * (s6 * s6 - s6);
+                            vir += (double)r2 * (double)fs;
+                        }
+                    }
+                }
+            }
+            if (ENERGY) {
+                block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_inter);
+                block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+            }
+        } else {
+            const int kk = k - ncc;
+            const int j0 = kk * kSolventChunk;
+            const int nsrc = min(kSolventChunk, a.cs_cnt[c] - j0);
+            for (int jj = p; jj < nsrc; jj += blockDim.x) {
+                float4 q = a.spos[a.cs_ids[(size_t)c * a.cap_cs + j0 + jj]];
+                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                src[jj] = q;
+            }
+            __syncthreads();
+            const float2 pr = a.tab_cs[r * kMaxTypes + tp];
+            if (active) {
+                for (int j = 0; j < nsrc; j++) {
+                    const float4 q = src[j];
+                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz, en, vir);
+                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz);
+                }
+            }
+            if (ENERGY) {
+                block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_cs);
+                block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+            }
+        }
+        if (active) {
+            const size_t slot = (size_t)w * kCompoundBlock + p;
+            if (FIXED) a.partC_fx[slot] = I3{to_fixed(lx), to_fixed(ly), to_fixed(lz)};
+            else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compoundKernel remainder (Engine.cu:918-931, 981-1056): bridge carry-in,
+// bonded terms (term forces computed once, then gathered per particle in term
+// index order = the serialized scatter's order, Engine.cu:765-771), intra-LJ
+// (geometric eps, exclusions), listed partials, integration, PBC.
+// ADVANCE=false computes forces (and energies) without changing the state.
+// ---------------------------------------------------------------------------
+template <bool ADVANCE, bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
+    __shared__ float4 pos[kCompoundBlock];
+    __shared__ F3 rb[kMaxBondsPerCompound * 2];
+    __shared__ F3 ra[kMaxAnglesPerCompound * 3];
+    __shared__ F3 rd[kMaxDihedralsPerCompound * 4];
+    __shared__ float2 tab[kMaxTypes * kMaxTypes];
+    __shared__ double red[2];
+    const int c = blockIdx.x;
+    const int p = threadIdx.x;
+    const int r = a.comp_rep[c];
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half;
+    const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
+    const bool active = p < n;
+    const int gi = base + (active ? p : 0);
+    float4 me = a.cpos[gi];
+    const float4 p0 = a.cpos[base];
+    hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);                 // Engine.cu:923
+    pos[p] = me;
+    for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) tab[t] = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + t];
+    __syncthreads();
+    double eb = 0, ea = 0, ed = 0;
+    const int b0 = a.bond_off[c], nb = a.bond_off[c + 1] - b0;
+    const int a0 = a.angle_off[c], na = a.angle_off[c + 1] - a0;
+    const int d0 = a.dih_off[c], nd = a.dih_off[c + 1] - d0;
+    auto P = [&](int i) { float4 q = pos[i]; return f3(q.x, q.y, q.z); };
+    for (int t = p; t < nb; t += blockDim.x) {
+        BondD bd = a.bonds[b0 + t];
+        pairbond(P(bd.a0), P(bd.a1), bd.b0, bd.kb, &rb[t * 2], &eb);
+    }
+    for (int t = p; t < na; t += blockDim.x) {
+        AngleD ad = a.angles[a0 + t];
+        anglebond(P(ad.a0), P(ad.a1), P(ad.a2), ad.t0, ad.k, &ra[t * 3], &ea);
+    }
+    if (a.enable_dihedrals) {
+        for (int t = p; t < nd; t += blockDim.x) {
+            DihD dd = a.dihs[d0 + t];
+            dihedral(P(dd.a0), P(dd.a1), P(dd.a2), P(dd.a3), dd.p0, dd.k, dd.n, &rd[t * 4], &ed);
+        }
+    }
+    __syncthreads();
+    float4 fb = a.fbridge[gi];
+    F3 force = f3(fb.x, fb.y, fb.z);                                        // Engine.cu:918
+    F3 bs = f3(0.f, 0.f, 0.f), as = bs, ds = bs;
+    if (active) {
+        for (int e = a.inc_off[gi]; e < a.inc_off[gi + 1]; e++) {
+            const uint32_t v = a.inc[e];
+            const uint32_t kind = v >> 30, slot = v & 3u, term = (v >> 2) & 0xFFFFFFFu;
+            if (kind == 0) bs = f3_add(bs, rb[term * 2 + slot]);
+            else if (kind == 1) as = f3_add(as, ra[term * 3 + slot]);
+            else ds = f3_add(ds, rd[term * 4 + slot]);
+        }
+    }
+    long long fx_i = 0, fy_i = 0, fz_i = 0;
+    if (FIXED) {
+        fx_i = to_fixed(force.x) + to_fixed(bs.x) + to_fixed(as.x) + (a.enable_dihedrals ? to_fixed(ds.x) : 0);
+        fy_i = to_fixed(force.y) + to_fixed(bs.y) + to_fixed(as.y) + (a.enable_dihedrals ? to_fixed(ds.y) : 0);
+        fz_i = to_fixed(force.z) + to_fixed(bs.z) + to_fixed(as.z) + (a.enable_dihedrals ? to_fixed(ds.z) : 0);
+    } else {
+        force = f3_add(force, bs);
+        force = f3_add(force, as);
+        if (a.enable_dihedrals) force = f3_add(force, ds);
+    }
+    // computeIntramolecularLJForces (Engine.cu:678-700)
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    double e_intra = 0, vir = 0;
+    if (active) {
+        const int tp = ftype(me);
+        const uint32_t* mk = a.intra_mask + (size_t)gi * 4;
+        const uint32_t m[4] = {mk[0], mk[1], mk[2], mk[3]};
+        for (int g = 0; g * 32 < n; g++) {
+            const uint32_t mw = g == 0 ? m[0] : g == 1 ? m[1] : g == 2 ? m[2] : m[3];
+            const int jend = min(n, g * 32 + 32);
+            for (int j = g * 32; j < jend; j++) {
+                const float4 q = pos[j];
+                const float2 pr = tab[ftype(q) * kMaxTypes + tp];
+                float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float inv = __builtin_amdgcn_rcpf(r2);
+                float s = pr.x * inv;
+                float s3 = s * s * s;
+                float fs = pr.y * s3 * inv * (1.0f - 2.0f * s3);
+                const bool ex = (j == p) || ((mw >> (j & 31)) & 1u);
+                fs = ex ? 0.0f : fs;
+                lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                if (ENERGY && !ex) {
+                    double sd = (double)pr.x / (double)r2, s6 = sd * sd * sd;
+                    e_intra += (double)pr.y / 6.0 * (s6 * s6 - s6);
+                    vir += (double)r2 * (double)fs;
+                }
+            }
+        }
+    }
+    if (FIXED) { fx_i += to_fixed(lx); fy_i += to_fixed(ly); fz_i += to_fixed(lz); }
+    else force = f3_add(force, f3(lx, ly, lz));
+    // listed partials: neighbour compounds (list order), then solvent chunks
+    if (active) {
+        const int s0 = a.cw_off[c], s1 = a.cw_off[c + 1];
+        for (int s = s0; s < s1; s++) {
+            const size_t slot = (size_t)s * kCompoundBlock + p;
+            if (FIXED) { I3 v = a.partC_fx[slot]; fx_i += v.x; fy_i += v.y; fz_i += v.z; }
+            else { float4 v = a.partC[slot]; force.x += v.x; force.y += v.y; force.z += v.z; }
+        }
+    }
+    if (FIXED) force = f3(from_fixed(fx_i), from_fixed(fy_i), from_fixed(fz_i));
+    if (active) a.cforce[gi] = make_float4(force.x, force.y, force.z, 0.f);
+    if (active && exact_len(force.x, force.y, force.z) > 2e12f) atomicOr(a.err, kErrCriticalForce);   // Engine.cu:1050
+    double ke = 0.0;
+    const double m = (double)a.mass[r * kMaxTypes + ftype(me)];
+    float4 pv = a.cprev[gi];
+    if (ADVANCE) {
+        float px = me.x, py = me.y, pz = me.z, qx = pv.x, qy = pv.y, qz = pv.z, dx, dy, dz;
+        const double mts = rp.dt / m * rp.dt;
+        integrate(px, py, pz, qx, qy, qz, force.x, force.y, force.z, mts, a.lambda[r], L, half, dx, dy, dz);
+        if (active) {
+            a.cprev[gi] = make_float4(qx, qy, qz, 0.f);
+            ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+        }
+        __syncthreads();
+        pos[p] = make_float4(px, py, pz, me.w);
+        __syncthreads();
+        // PBC on particle 0, then re-image every particle against it (Engine.cu:993-996)
+        float4 z = pos[0];
+        float zx = pbc1(z.x, L), zy = pbc1(z.y, L), zz = pbc1(z.z, L);
+        hyperpos(zx, zy, zz, px, py, pz, L, half);
+        if (active) a.cpos_next[gi] = make_float4(px, py, pz, me.w);
+    } else if (active) {
+        float qx = pv.x, qy = pv.y, qz = pv.z;
+        hyperpos(me.x, me.y, me.z, qx, qy, qz, L, half);
+        float dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
+        ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+    }
+    // kinetic-energy partial of this compound (FP64, fixed order)
+    double ks = wave_sum(ke);
+    if ((p & 63) == 0) red[p >> 6] = ks;
+    __syncthreads();
+    if (p == 0) {
+        double tot = red[0] + (blockDim.x > 64 ? red[1] : 0.0);
+        if (ADVANCE) a.ke_comp[c] = tot;
+        if (ENERGY) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
+    }
+    if (ENERGY) {
+        block_atomic_energy(eb, a.energy + r * kEnergyTerms + kE_bond);
+        block_atomic_energy(ea, a.energy + r * kEnergyTerms + kE_angle);
+        block_atomic_energy(ed, a.energy + r * kEnergyTerms + kE_dih);
+        block_atomic_energy(e_intra, a.energy + r * kEnergyTerms + kE_intra);
+        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solventForceKernel remainder (Engine.cu:1121-1155): solvent-solvent LJ over
+// the solvent's list (the self position is re-imaged against every neighbour,
+// Engine.cu:710), integrate, PBC, NaN check.  One thread per solvent; tiles
+// never straddle replicas.
+// ---------------------------------------------------------------------------
+template <bool ADVANCE, bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
+    __shared__ double red[4];
+    const int4 tile = a.stiles[blockIdx.x];
+    const int r = tile.x;
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half;
+    const int s = tile.y + threadIdx.x;
+    const bool active = s < tile.z;
+    const int si = active ? s : tile.y;
+    const float4 me = a.spos[si];
+    float sx = me.x, sy = me.y, sz = me.z;
+    for (int c = rp.comp_begin; c < rp.comp_end; c++) {                    // Engine.cu:1111
+        float4 q = a.cpos[a.comp_off[c]];
+        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
+    }
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    long long ix = 0, iy = 0, iz = 0;
+    for (int k = 0; k < rp.chunk_count; k++) {
+        const size_t slot = (size_t)k * a.n_solv + si;
+        if (FIXED) { I3 v = a.partS_fx[slot]; ix += v.x; iy += v.y; iz += v.z; }
+        else { float4 v = a.partS[slot]; fx += v.x; fy += v.y; fz += v.z; }
+    }
+    const float2 pss = a.tab_ss[r];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    double en = 0, vir = 0;
+    const int cnt = active ? a.ss_cnt[si] : 0;
+    for (int k = 0; k < cnt; k++) {
+        const float4 q = a.spos[a.ss_ids[(size_t)k * a.n_solv + si]];
+        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
+        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
+        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
+    }
+    if (FIXED) {
+        ix += to_fixed(lx); iy += to_fixed(ly); iz += to_fixed(lz);
+        fx = from_fixed(ix); fy = from_fixed(iy); fz = from_fixed(iz);
+    } else {
+        fx += lx; fy += ly; fz += lz;
+    }
+    if (active) a.sforce[s] = make_float4(fx, fy, fz, 0.f);
+    const double m = (double)a.mass[r * kMaxTypes + 0];
+    double ke = 0.0;
+    float4 pv = a.sprev[si];
+    if (ADVANCE) {
+        float qx = pv.x, qy = pv.y, qz = pv.z, dx, dy, dz;
+        const double mts = rp.dt / m * rp.dt;
+        integrate(sx, sy, sz, qx, qy, qz, fx, fy, fz, mts, a.lambda[r], L, half, dx, dy, dz);
+        sx = pbc1(sx, L); sy = pbc1(sy, L); sz = pbc1(sz, L);
+        if (active) {
+            if (sx != sx) atomicOr(a.err, kErrNaN);                        // Engine.cu:1151
+            a.spos_next[s] = make_float4(sx, sy, sz, 0.f);
+            a.sprev[s] = make_float4(qx, qy, qz, 0.f);
+            ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+        }
+    } else if (active) {
+        float qx = pv.x, qy = pv.y, qz = pv.z;
+        hyperpos(me.x, me.y, me.z, qx, qy, qz, L, half);
+        float dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
+        ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+    }
+    double ks = wave_sum(ke);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ks;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0;
+        for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
+        if (ADVANCE) a.ke_stile[tile.w] = tot;
+        if (ENERGY) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
+    }
+    if (ENERGY) {
+        block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_ss);
+        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thermostat (Engine::handleBoxtemp, Engine.cu:362-393, made a device FP64
+// reduction): per replica, T = sum m|dx|^2 / dt^2 / (3 N R); Berendsen
+// lambda = sqrt(1 + dt/tau (T0/T - 1)) with the reference's [0.9, 1.1] clamp
+// and T=0 guard.  One wavefront per replica, fixed summation order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps, int n_rep, const double* ke_comp,
+                                                      const double* ke_stile, float* lambda, float* lambda_applied,
+                                                      double* temperature, int mode) {
+    const int r = blockIdx.x;
+    if (r >= n_rep) return;
+    const ReplicaInfo rp = reps[r];
+    double s = 0.0;
+    for (int c = rp.comp_begin + threadIdx.x; c < rp.comp_end; c += 64) s += ke_comp[c];
+    for (int t = rp.stile_begin + threadIdx.x; t < rp.stile_end; t += 64) s += ke_stile[t];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) {
+        lambda_applied[r] = lambda[r];
+        const double ntot = (double)((rp.chain_end - rp.chain_begin) + (rp.solv_end - rp.solv_begin));
+        double temp = ntot > 0 ? s / (rp.dt * rp.dt) / (3.0 * ntot * 8.3145) : 0.0;
+        temperature[r] = temp;
+        float lam = 1.0f;
+        if (mode == 1) {
+            double t = temp == 0.0 ? 1.0 : temp;
+            double l = sqrt(1.0 + rp.dt / rp.tau * ((double)rp.T0 / t - 1.0));
+            l = l < 1.1 ? l : 1.1;
+            l = l > 0.9 ? l : 0.9;
+            lam = (float)l;
+        }
+        lambda[r] = lam;
+    }
+}
+
+// ===========================================================================
+// Neighbour lists (Engine::updateNeighborLists, Engine.cu:208-287).
+// The reference's incremental cull/add with hash dedupe yields, per pair:
+//   compound-compound (lo < hi): member = (was && d < C+((r_lo+r_hi)+C)) || d < C+((r_lo+r_lo)+C)
+//                                 (the add threshold reads the lower id's radius twice, Engine.cu:243)
+//   compound-solvent:            member = d < C + (r_c + C)
+//   solvent-solvent (lo < hi):   member = d < C + C
+// with d = |a - hyperpos(a, b)| in FP32, a = the lower id (or the compound key).
+// Lists are produced in ascending id order (compound lists) or cell-scan order
+// (solvent lists); the sets are bit-exact, the order is this build's own.
+// ===========================================================================
+struct NlistArgs {
+    const ReplicaInfo* reps;
+    const float4* cpos;
+    const float4* spos;
+    int n_comp, n_solv;
+    const int* comp_off; const int* comp_rep; const int* comp_key; const float* comp_sphere;
+    const int* solv_rep;
+    float cutoff;
+    int cap_cc, cap_cs, cap_ss;
+    int* cc_cnt; int* cc_ids; int* cc_old_cnt; int* cc_old_ids;
+    int* cs_cnt; int* cs_ids;
+    int* ss_cnt; int* ss_ids;
+    int n_cells;
+    int* cell_cnt; int* cell_start; int* cell_cursor; int* cell_ids; int* solv_cell;
+    int* cw_off; int* cw_total;
+    uint32_t* err;
+    int enable_solvents;
+};
+
+LIMA_DEV float4 key_pos(const NlistArgs& a, int c) { return a.cpos[a.comp_off[c] + a.comp_key[c]]; }
+
+__global__ void nl_copy_cc_kernel(NlistArgs a) {
+    const int c = blockIdx.x;
+    const int cnt = a.cc_cnt[c];
+    if (threadIdx.x == 0) a.cc_old_cnt[c] = cnt;
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) a.cc_old_ids[(size_t)c * a.cap_cc + k] = a.cc_ids[(size_t)c * a.cap_cc + k];
+}
+
+// one wavefront per compound, candidates in ascending id order
+__global__ __launch_bounds__(64) void nl_compound_compound_kernel(NlistArgs a) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int r = a.comp_rep[i];
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half, C = a.cutoff;
+    const float4 ki = key_pos(a, i);
+    const float ri = a.comp_sphere[i];
+    const int old_cnt = a.cc_old_cnt[i];
+    const int* old = a.cc_old_ids + (size_t)i * a.cap_cc;
+    int count = 0;
+    for (int j0 = rp.comp_begin; j0 < rp.comp_end; j0 += 64) {
+        const int j = j0 + lane;
+        bool member = false;
+        if (j < rp.comp_end && j != i) {
+            const int lo = min(i, j), hi = max(i, j);
+            const float4 klo = lo == i ? ki : key_pos(a, j);
+            const float4 khi = hi == i ? ki : key_pos(a, j);
+            const float rlo = a.comp_sphere[lo], rhi = a.comp_sphere[hi];
+            bool was = false;
+            for (int e = 0; e < old_cnt; e++) was |= (old[e] == j);
+            const bool keep = within(klo.x, klo.y, klo.z, khi.x, khi.y, khi.z, fadd(fadd(rlo, rhi), C), C, L, half);
+            const bool add = within(klo.x, klo.y, klo.z, khi.x, khi.y, khi.z, fadd(fadd(rlo, rlo), C), C, L, half);
+            member = (was && keep) || add;
+        }
+        const unsigned long long ball = __ballot(member);
+        const int before = __popcll(ball & ((1ull << lane) - 1ull));
+        if (member) {
+            const int pos = count + before;
+            if (pos < a.cap_cc) a.cc_ids[(size_t)i * a.cap_cc + pos] = j;
+        }
+        count += __popcll(ball);
+    }
+    (void)ri;
+    if (lane == 0) {
+        if (count > a.cap_cc) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 1, count); }
+        a.cc_cnt[i] = min(count, a.cap_cc);
+    }
+}
+
+// one block (256) per compound, solvents of the replica in ascending order
+__global__ __launch_bounds__(256) void nl_compound_solvent_kernel(NlistArgs a) {
+    __shared__ int wave_cnt[4];
+    const int i = blockIdx.x;
+    const int r = a.comp_rep[i];
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half, C = a.cutoff;
+    const float4 ki = key_pos(a, i);
+    const float off = fadd(a.comp_sphere[i], C);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int count = 0;
+    for (int s0 = rp.solv_begin; s0 < rp.solv_end; s0 += 256) {
+        const int s = s0 + threadIdx.x;
+        bool member = false;
+        if (s < rp.solv_end) {
+            const float4 q = a.spos[s];
+            member = within(ki.x, ki.y, ki.z, q.x, q.y, q.z, off, C, L, half);
+        }
+        const unsigned long long ball = __ballot(member);
+        if (lane == 0) wave_cnt[wid] = __popcll(ball);
+        __syncthreads();
+        int before = __popcll(ball & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wid; w++) before += wave_cnt[w];
+        if (member) {
+            const int pos = count + before;
+            if (pos < a.cap_cs) a.cs_ids[(size_t)i * a.cap_cs + pos] = s;
+        }
+        count += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (count > a.cap_cs) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 2, count); }
+        a.cs_cnt[i] = min(count, a.cap_cs);
+    }
+}
+
+LIMA_DEV int cell_coord(float v, float inv, int n) {
+    int c = (int)(v * inv);
+    c = c < 0 ? 0 : c;
+    return c >= n ? n - 1 : c;
+}
+
+__global__ void nl_cell_zero_kernel(int* cell_cnt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cell_cnt[i] = 0;
+}
+
+__global__ void nl_cell_count_kernel(NlistArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n_solv) return;
+    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
+    const float4 q = a.spos[s];
+    const int n = rp.ncell;
+    const int cx = cell_coord(q.x, rp.cell_inv, n), cy = cell_coord(q.y, rp.cell_inv, n), cz = cell_coord(q.z, rp.cell_inv, n);
+    const int cell = rp.cell_begin + (cx * n + cy) * n + cz;
+    a.solv_cell[s] = cell;
+    atomicAdd(&a.cell_cnt[cell], 1);
+}
+
+// exclusive scan of cell counts (single block of 1024)
+__global__ __launch_bounds__(1024) void nl_cell_scan_kernel(NlistArgs a) {
+    __shared__ int part[1024];
+    const int n = a.n_cells;
+    const int per = (n + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    int s = 0;
+    for (int k = 0; k < per && b + k < n; k++) s += a.cell_cnt[b + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int k = 0; k < per && b + k < n; k++) {
+        a.cell_start[b + k] = run;
+        a.cell_cursor[b + k] = run;
+        run += a.cell_cnt[b + k];
+    }
+    if (threadIdx.x == 1023) a.cell_start[n] = part[1023];
+}
+
+__global__ void nl_cell_scatter_kernel(NlistArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n_solv) return;
+    const int pos = atomicAdd(&a.cell_cursor[a.solv_cell[s]], 1);
+    a.cell_ids[pos] = s;
+}
+
+// sort each cell's ids ascending (rank sort in LDS), making the lists deterministic
+__global__ __launch_bounds__(256) void nl_cell_sort_kernel(NlistArgs a) {
+    constexpr int CAP = 2048;
+    __shared__ int ids[CAP];
+    const int cell = blockIdx.x;
+    const int b = a.cell_start[cell], n = a.cell_start[cell + 1] - b;
+    if (n > CAP) {
+        if (threadIdx.x == 0) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, n); }
+        return;
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) ids[k] = a.cell_ids[b + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int v = ids[k];
+        int rank = 0;
+        for (int e = 0; e < n; e++) rank += ids[e] < v;
+        a.cell_ids[b + rank] = v;
+    }
+}
+
+// one thread per solvent: scan the (deduplicated) 27 neighbouring cells
+__global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n_solv) return;
+    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
+    const float L = rp.L, half = rp.half, C = a.cutoff;
+    const float4 me = a.spos[s];
+    const int n = rp.ncell;
+    const int cx = cell_coord(me.x, rp.cell_inv, n), cy = cell_coord(me.y, rp.cell_inv, n), cz = cell_coord(me.z, rp.cell_inv, n);
+    const int span = n >= 3 ? 3 : n;
+    const int lo_off = n >= 3 ? -1 : 0;
+    int count = 0;
+    for (int ox = 0; ox < span; ox++) {
+        const int x = (n >= 3) ? (cx + lo_off + ox + n) % n : ox;
+        for (int oy = 0; oy < span; oy++) {
+            const int y = (n >= 3) ? (cy + lo_off + oy + n) % n : oy;
+            for (int oz = 0; oz < span; oz++) {
+                const int z = (n >= 3) ? (cz + lo_off + oz + n) % n : oz;
+                const int cell = rp.cell_begin + (x * n + y) * n + z;
+                const int e1 = a.cell_start[cell + 1];
+                for (int e = a.cell_start[cell]; e < e1; e++) {
+                    const int j = a.cell_ids[e];
+                    if (j == s) continue;
+                    const float4 q = a.spos[j];
+                    bool member;
+                    if (s < j) member = within(me.x, me.y, me.z, q.x, q.y, q.z, C, C, L, half);
+                    else member = within(q.x, q.y, q.z, me.x, me.y, me.z, C, C, L, half);
+                    if (member) {
+                        if (count < a.cap_ss) a.ss_ids[(size_t)count * a.n_solv + s] = j;
+                        count++;
+                    }
+                }
+            }
+        }
+    }
+    if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
+    a.ss_cnt[s] = min(count, a.cap_ss);
+}
+
+// per compound work slots: listed neighbours + ceil(solvent list / 256); exclusive scan
+__global__ __launch_bounds__(1024) void nl_work_kernel(NlistArgs a) {
+    __shared__ int part[1024];
+    const int n = a.n_comp;
+    const int per = (n + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    auto slots = [&](int c) {
+        int s = a.cc_cnt[c];
+        if (a.enable_solvents) s += (a.cs_cnt[c] + kSolventChunk - 1) / kSolventChunk;
+        return s;
+    };
+    int s = 0;
+    for (int k = 0; k < per && b + k < n; k++) s += slots(b + k);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int k = 0; k < per && b + k < n; k++) {
+        a.cw_off[b + k] = run;
+        run += slots(b + k);
+    }
+    if (threadIdx.x == 1023) { a.cw_off[n] = part[1023]; *a.cw_total = part[1023]; }
+}
+
+}  // namespace lima

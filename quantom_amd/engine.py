@@ -1,0 +1,258 @@
+"""Python binding of the gfx950 engine over the C ABI (include/lima.h).
+
+`Engine` keeps the reference's step API (Engine.cuh:109-208):
+`Engine(simulation, params)` builds the device state and the first neighbour
+lists (Engine.cu:9-38); `deviceMaster()` runs one step (Engine.cu:44-47,
+400-451); `hostMaster()` is the reference's per-step host hook (Engine.cu:49-75),
+whose work (thermostat, neighbour-list rebuild) now runs on the device inside
+the step, so here it only surfaces errors.  A `Simulation` holds one or more
+systems (replicas) stepped together.
+
+The native library must be present: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (CEnergies, CRunParams, CSystem, LimaSystem, RunParams, STATUS_NAMES, LIMA_OK)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblima_hip.so")
+_lib = None
+
+EXPORTS = (
+    "lima_version", "lima_device_count", "lima_create", "lima_destroy", "lima_last_error", "lima_step",
+    "lima_step_async", "lima_sync", "lima_rebuild_neighbors", "lima_get_state", "lima_set_state",
+    "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
+    "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_get_config",
+    "lima_autotune",
+)
+
+
+class LimaStatusError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library() -> C.CDLL:
+    """Load liblima_hip.so; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `python -m quantom_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    L.lima_version.restype = C.c_char_p
+    L.lima_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.lima_create.argtypes = [C.POINTER(CSystem), C.c_int32, C.POINTER(CRunParams), C.POINTER(vp)]
+    L.lima_destroy.argtypes = [vp]
+    L.lima_last_error.restype = C.c_char_p
+    L.lima_last_error.argtypes = [vp]
+    for name in ("lima_step", "lima_step_async", "lima_autotune"):
+        getattr(L, name).argtypes = [vp, C.c_int32]
+    L.lima_sync.argtypes = [vp]
+    L.lima_rebuild_neighbors.argtypes = [vp]
+    L.lima_get_state.argtypes = [vp, fp, fp, fp, fp]
+    L.lima_set_state.argtypes = [vp, fp, fp, fp, fp]
+    L.lima_get_forces.argtypes = [vp, fp, fp]
+    L.lima_compute_forces.argtypes = [vp, fp, fp]
+    L.lima_get_energies.argtypes = [vp, C.POINTER(CEnergies)]
+    L.lima_get_neighbor_counts.argtypes = [vp, ip, ip, ip]
+    L.lima_get_neighbors.argtypes = [vp, ip, ip, ip]
+    L.lima_set_neighbors.argtypes = [vp, ip, ip, ip, ip, ip, ip]
+    L.lima_get_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    L.lima_get_config.argtypes = [vp, ip]
+    L.lima_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+class Simulation:
+    """Host description of the systems stepped together (the reference's Simulation, Simulation.cuh:97-178)."""
+
+    def __init__(self, systems, params: RunParams | None = None):
+        self.systems = list(systems) if isinstance(systems, (list, tuple)) else [systems]
+        self.params = params or RunParams()
+        self.n_chain = sum(s.n_chain for s in self.systems)
+        self.n_solvents = sum(s.n_solvents for s in self.systems)
+        self.n_compounds = sum(s.n_compounds for s in self.systems)
+        self.chain_begin = np.cumsum([0] + [s.n_chain for s in self.systems])
+        self.solvent_begin = np.cumsum([0] + [s.n_solvents for s in self.systems])
+        self.compound_begin = np.cumsum([0] + [s.n_compounds for s in self.systems])
+
+
+class Engine:
+    """One engine handle (single-threaded use)."""
+
+    def __init__(self, simulation: Simulation | LimaSystem | list, params: RunParams | None = None):
+        if not isinstance(simulation, Simulation):
+            simulation = Simulation(simulation, params)
+        self.simulation = simulation
+        self.params = simulation.params
+        self.L = load_library()
+        arr = (CSystem * len(simulation.systems))()
+        self._keep = []
+        for k, s in enumerate(simulation.systems):
+            arr[k] = s.to_c()
+            self._keep.append(s._keep)
+        self._cparams = self.params.to_c()
+        h = C.c_void_p()
+        st = self.L.lima_create(arr, len(simulation.systems), C.byref(self._cparams), C.byref(h))
+        if st != LIMA_OK:
+            raise LimaStatusError(st, self.L.lima_last_error(None).decode())
+        self.h = h
+        self.steps = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lima_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st != LIMA_OK:
+            raise LimaStatusError(st, self.L.lima_last_error(self.h).decode())
+
+    # ---- the reference's step API ----
+    def deviceMaster(self):
+        """One step on the device (Engine::deviceMaster, Engine.cu:44-47)."""
+        self.step(1)
+
+    def hostMaster(self):
+        """Per-step host hook (Engine::hostMaster, Engine.cu:49-75); the work runs on the device."""
+        self.sync()
+
+    def step(self, n: int = 1):
+        self._check(self.L.lima_step(self.h, n))
+        self.steps += n
+
+    def step_async(self, n: int):
+        self._check(self.L.lima_step_async(self.h, n))
+        self.steps += n
+
+    def sync(self):
+        self._check(self.L.lima_sync(self.h))
+
+    def rebuild_neighbors(self):
+        self._check(self.L.lima_rebuild_neighbors(self.h))
+
+    # ---- state ----
+    def get_state(self):
+        sim = self.simulation
+        cx = np.zeros((sim.n_chain, 3), np.float32); cxm = np.zeros_like(cx)
+        sx = np.zeros((sim.n_solvents, 3), np.float32); sxm = np.zeros_like(sx)
+        self._check(self.L.lima_get_state(self.h, _f(cx), _f(cxm), _f(sx), _f(sxm)))
+        return cx, cxm, sx, sxm
+
+    def set_state(self, cx, cxm, sx, sxm):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (cx, cxm, sx, sxm)]
+        self._check(self.L.lima_set_state(self.h, *[_f(v) for v in a]))
+
+    def forces(self):
+        sim = self.simulation
+        cf = np.zeros((sim.n_chain, 3), np.float32); sf = np.zeros((sim.n_solvents, 3), np.float32)
+        self._check(self.L.lima_get_forces(self.h, _f(cf), _f(sf)))
+        return cf, sf
+
+    def compute_forces(self):
+        sim = self.simulation
+        cf = np.zeros((sim.n_chain, 3), np.float32); sf = np.zeros((sim.n_solvents, 3), np.float32)
+        self._check(self.L.lima_compute_forces(self.h, _f(cf), _f(sf)))
+        return cf, sf
+
+    def energies(self) -> list:
+        n = len(self.simulation.systems)
+        arr = (CEnergies * n)()
+        self._check(self.L.lima_get_energies(self.h, arr))
+        return [arr[k].as_dict() for k in range(n)]
+
+    # ---- neighbour lists (global ids within the concatenation) ----
+    def neighbor_counts(self):
+        sim = self.simulation
+        cc = np.zeros(max(sim.n_compounds, 1), np.int32); cs = np.zeros_like(cc)
+        ss = np.zeros(max(sim.n_solvents, 1), np.int32)
+        self._check(self.L.lima_get_neighbor_counts(self.h, _i(cc), _i(cs), _i(ss)))
+        return cc[:sim.n_compounds], cs[:sim.n_compounds], ss[:sim.n_solvents]
+
+    def neighbors(self):
+        """Returns (compound->compound, compound->solvent, solvent->solvent) lists of global ids."""
+        sim, p = self.simulation, self.params
+        ccn, csn, ssn = self.neighbor_counts()
+        cci = np.zeros((max(sim.n_compounds, 1), p.max_compound_neighbors), np.int32)
+        csi = np.zeros((max(sim.n_compounds, 1), p.max_compound_solvents), np.int32)
+        ssi = np.zeros((max(sim.n_solvents, 1), p.max_solvent_neighbors), np.int32)
+        self._check(self.L.lima_get_neighbors(self.h, _i(cci), _i(csi), _i(ssi)))
+        return ([cci[c, :ccn[c]].copy() for c in range(sim.n_compounds)],
+                [csi[c, :csn[c]].copy() for c in range(sim.n_compounds)],
+                [ssi[s, :ssn[s]].copy() for s in range(sim.n_solvents)])
+
+    def set_neighbors(self, cc=None, cs=None, ss=None):
+        sim, p = self.simulation, self.params
+
+        def pack(lists, n, cap):
+            if lists is None:
+                return None, None
+            cnt = np.array([len(l) for l in lists] or [0], np.int32)
+            ids = np.zeros((max(n, 1), cap), np.int32)
+            for k, l in enumerate(lists):
+                ids[k, :min(len(l), cap)] = l[:cap]
+            return cnt, ids
+
+        a = pack(cc, sim.n_compounds, p.max_compound_neighbors)
+        b = pack(cs, sim.n_compounds, p.max_compound_solvents)
+        c = pack(ss, sim.n_solvents, p.max_solvent_neighbors)
+        self._check(self.L.lima_set_neighbors(self.h, _i(a[0]), _i(a[1]), _i(b[0]), _i(b[1]), _i(c[0]), _i(c[1])))
+
+    def timings(self) -> dict:
+        t = (C.c_double * 5)()
+        self._check(self.L.lima_get_timings(self.h, t))
+        return {"device_us": t[0], "rebuild_us": t[1], "host_us": t[2], "steps": int(t[3]), "rebuilds": int(t[4])}
+
+    KERNELS = ("solvent_chain", "compound_listed", "compound_integrate", "solvent_integrate", "bridge", "rebuild")
+
+    def time_kernels(self, reps: int = 20) -> dict:
+        """Average microseconds per launch of each step kernel, launched alone (HIP events)."""
+        t = (C.c_double * 6)()
+        self._check(self.L.lima_time_kernels(self.h, reps, t))
+        return dict(zip(self.KERNELS, list(t)))
+
+    def config(self) -> dict:
+        c = (C.c_int32 * 8)()
+        self._check(self.L.lima_get_config(self.h, c))
+        keys = ("solvents_per_thread", "chunk_particles", "solvent_chain_items", "max_chunks", "use_graph",
+                "n_cells", "n_solvent_tiles", "n_replicas")
+        return dict(zip(keys, list(c)))
+
+    def autotune(self, trial_steps: int = 20) -> dict:
+        self._check(self.L.lima_autotune(self.h, trial_steps))
+        return self.config()
+
+
+def version() -> str:
+    return load_library().lima_version().decode()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    load_library().lima_device_count(C.byref(n))
+    return n.value

@@ -1,0 +1,389 @@
+"""Seeded synthetic systems for the benchmark workloads.
+
+This replaces the reference's setup path (BoxBuilder.cu:5-145, 248-276;
+CompoundBuilder.cpp:7-49, 225-318) with a generator of chains and single-site
+solvent, producing the same data the reference's step consumes:
+
+* chains are self-avoiding random walks (bond 0.15 nm, bond angle 109.5 deg,
+  uniform torsion), cut into compounds of 128 particles; a placement is
+  rejected when closer than 0.30 nm to any bead outside its 1-2/1-3/1-4 set;
+* compound particles are stored re-imaged around their particle 0, with
+  particle 0 inside the box (as the compound kernel leaves them,
+  Engine.cu:993-996); key particle and confining sphere follow
+  Compound::calcParticleSphere (Bodies.cuh:407-423);
+* bonded terms inside a compound use compound-local indices; terms spanning
+  two compounds go to the bridge between them, with bridge-local indices and
+  particles added in first-seen order (CompoundBridge::addBondParticles,
+  Bodies.cuh:551-557);
+* every bonded tuple adds mutual LJ exclusions (distributeLJIgnores,
+  CompoundBuilder.cpp:307-318);
+* solvent sits on a jittered cubic lattice at 33 nm^-3, skipping sites within
+  0.23 nm of a bead; velocities are Maxwell at T0 and prev = x - v dt.
+Type 0 is the solvent type (ATOMTYPE_SOL, Forcefield.cuh:15).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .abi import ANGLE_DTYPE, DIHEDRAL_DTYPE, PAIR_DTYPE, LimaSystem
+
+GAS_R = 8.3145                   # J/(mol K), Engine.cu:356
+BOND_B0 = 0.15                   # nm
+BOND_KB = 2.5e8                  # J/mol/nm^2  (2.5e5 kJ/mol/nm^2)
+ANGLE_THETA0 = float(np.float32(math.radians(109.5)))
+ANGLE_K = 4.0e5                  # J/mol/rad^2 (400 kJ/mol/rad^2)
+DIHEDRAL_K = 1000.0              # J/mol
+DIHEDRAL_N = 3
+DIHEDRAL_PHI0 = 0.0
+SOLVENT_PARAMS = (0.018, 0.3166, 650.0)   # kg/mol, nm, J/mol
+EXCLUSION_DIST = 0.30            # nm, bead-bead self-avoidance of the chain walk
+# Lattice sites closer than this to a bead are skipped.  0.23 nm reproduces the
+# stated solvent counts of the workloads at 33 nm^-3 (~24,000 in the 10 nm box
+# with 12,800 beads, ~160,000 in the 18 nm box with 51,200 beads).
+SOLVENT_EXCLUSION = 0.23         # nm
+DT = 2e-6                        # ns (2 fs)
+COMPOUND_SIZE = 128
+
+
+def make_forcefield(rng: np.random.Generator, n_types: int) -> np.ndarray:
+    ff = np.zeros((32, 3), dtype=np.float32)
+    ff[:, :] = (0.012, 0.32, 100.0)          # unused slots: benign values
+    ff[0] = SOLVENT_PARAMS
+    ff[1:n_types + 1, 0] = rng.uniform(12.0, 16.0, n_types) * 1e-3
+    ff[1:n_types + 1, 1] = rng.uniform(0.30, 0.35, n_types)
+    ff[1:n_types + 1, 2] = rng.uniform(0.1, 0.5, n_types) * 1000.0
+    return ff
+
+
+class _Grid:
+    """Periodic cell hash for the self-avoidance test."""
+
+    def __init__(self, L: float, h: float):
+        self.L = L
+        self.n = max(1, int(L // h))
+        self.cells: dict = {}
+
+    def key(self, p):
+        q = np.floor(np.mod(p, self.L) / self.L * self.n).astype(int) % self.n
+        return (int(q[0]), int(q[1]), int(q[2]))
+
+    def add(self, idx, p):
+        self.cells.setdefault(self.key(p), []).append(idx)
+
+    def remove(self, idx, p):
+        self.cells[self.key(p)].remove(idx)
+
+    def near(self, p):
+        cx, cy, cz = self.key(p)
+        out = []
+        rng = (-1, 0, 1) if self.n >= 3 else tuple(range(self.n))
+        seen = set()
+        for dx in rng:
+            for dy in rng:
+                for dz in rng:
+                    k = ((cx + dx) % self.n, (cy + dy) % self.n, (cz + dz) % self.n)
+                    if k in seen:
+                        continue
+                    seen.add(k)
+                    out.extend(self.cells.get(k, ()))
+        return out
+
+
+def _place(a, b, c, bond, angle, torsion):
+    """NeRF: position of d given a, b, c, |cd|, angle bcd and torsion abcd."""
+    bc = c - b
+    bc /= np.linalg.norm(bc)
+    n = np.cross(b - a, bc)
+    nn = np.linalg.norm(n)
+    if nn < 1e-9:
+        n = np.cross(bc, np.array([1.0, 0.0, 0.0]))
+        if np.linalg.norm(n) < 1e-9:
+            n = np.cross(bc, np.array([0.0, 1.0, 0.0]))
+        nn = np.linalg.norm(n)
+    n /= nn
+    m = np.cross(n, bc)
+    d2 = np.array([-bond * math.cos(angle), bond * math.sin(angle) * math.cos(torsion),
+                   bond * math.sin(angle) * math.sin(torsion)])
+    return c + d2[0] * bc + d2[1] * m + d2[2] * n
+
+
+def _random_unit(rng):
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def _walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size):
+    """Self-avoiding walks (unwrapped coordinates), one per chain."""
+    pos = np.zeros((n_chains * n_beads_chain, 3))
+    grid = _Grid(L, EXCLUSION_DIST)
+    excl2 = EXCLUSION_DIST ** 2
+
+    def ok(p, gidx, chain_start, comp_first):
+        if confine is not None and comp_first is not None:
+            if np.linalg.norm(p - comp_first) > confine:
+                return False
+        cand = [j for j in grid.near(p) if not (chain_start <= j and gidx - 3 <= j < gidx)]
+        if not cand:
+            return True
+        d = pos[cand] - p
+        d -= L * np.round(d / L)
+        return bool(np.min(np.einsum("ij,ij->i", d, d)) >= excl2)
+
+    for ch in range(n_chains):
+        start = ch * n_beads_chain
+        restarts = 0
+        while True:
+            placed = 0
+            fails = 0
+            while placed < n_beads_chain:
+                g = start + placed
+                comp_first = None
+                if placed % compound_size != 0:
+                    comp_first = pos[start + (placed // compound_size) * compound_size]
+                success = False
+                for _ in range(40):
+                    if placed == 0:
+                        p = rng.uniform(0, L, 3)
+                    elif placed == 1:
+                        p = pos[g - 1] + BOND_B0 * _random_unit(rng)
+                    elif placed == 2:
+                        a = pos[g - 1] + _random_unit(rng)
+                        p = _place(a, pos[g - 2], pos[g - 1], BOND_B0, ANGLE_THETA0, rng.uniform(0, 2 * math.pi))
+                    else:
+                        p = _place(pos[g - 3], pos[g - 2], pos[g - 1], BOND_B0, ANGLE_THETA0,
+                                   rng.uniform(0, 2 * math.pi))
+                    if ok(p, g, start, comp_first):
+                        success = True
+                        break
+                if success:
+                    pos[g] = p
+                    grid.add(g, p)
+                    placed += 1
+                    continue
+                fails += 1
+                if fails > 200 or placed < 4:
+                    break
+                back = min(placed - 1, 4 + int(rng.integers(0, 6)))
+                for _ in range(back):
+                    placed -= 1
+                    grid.remove(start + placed, pos[start + placed])
+            if placed == n_beads_chain:
+                break
+            for k in range(placed):
+                grid.remove(start + k, pos[start + k])
+            restarts += 1
+            if restarts > 50:
+                raise RuntimeError("could not place chain; box too crowded")
+    return pos
+
+
+def _compound_sphere(p: np.ndarray):
+    """Compound::calcParticleSphere (Bodies.cuh:407-423) in FP32."""
+    p = p.astype(np.float32)
+    n = len(p)
+    com = np.zeros(3, dtype=np.float32)
+    inv = np.float32(1.0) / np.float32(n)
+    for i in range(n):
+        com = (com + (p[i] * inv).astype(np.float32)).astype(np.float32)
+    d = np.sqrt(((p - com).astype(np.float32) ** 2).sum(axis=1, dtype=np.float32)).astype(np.float32)
+    return int(np.argmin(d)), float(d.max())
+
+
+def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: float, n_types: int,
+                 n_solvents: int | None, T0: float = 300.0, dt: float = DT,
+                 solvent_density: float = 33.0, compound_size: int = COMPOUND_SIZE,
+                 forcefield: np.ndarray | None = None) -> LimaSystem:
+    """Build one seeded system.  n_solvents=None keeps every free lattice site."""
+    rng = np.random.default_rng(seed)
+    ff = make_forcefield(rng, n_types) if forcefield is None else forcefield.astype(np.float32)
+    L = float(box_len)
+    n_beads_chain = compounds_per_chain * compound_size
+    confine = 0.38 * L
+    walk = (_walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size) if n_chains > 0
+            else np.zeros((0, 3)))
+    n_chain = len(walk)
+    types = rng.integers(1, n_types + 1, size=n_chain).astype(np.uint8)
+    masses = ff[types, 0].astype(np.float64)
+
+    nc = n_chains * compounds_per_chain
+    coff = np.arange(nc + 1, dtype=np.int32) * compound_size
+    cpos = np.zeros((n_chain, 3), dtype=np.float32)
+    for c in range(nc):
+        seg = walk[coff[c]:coff[c + 1]]
+        p0 = np.mod(seg[0], L)
+        cpos[coff[c]:coff[c + 1]] = (p0 + (seg - seg[0])).astype(np.float32)
+    key = np.zeros(nc, dtype=np.int32)
+    sphere = np.zeros(nc, dtype=np.float32)
+    for c in range(nc):
+        key[c], sphere[c] = _compound_sphere(cpos[coff[c]:coff[c + 1]])
+
+    # ---- topology: compound terms, bridges, exclusions ----
+    comp_of = np.arange(n_chain) // compound_size
+    local_of = np.arange(n_chain) % compound_size
+    c_bonds = [[] for _ in range(nc)]
+    c_angles = [[] for _ in range(nc)]
+    c_dihs = [[] for _ in range(nc)]
+    bridges = {}   # (left, right) -> dict(particles=[global ids], bonds, angles, dihs)
+    ignores = [[] for _ in range(n_chain)]
+
+    def add_term(kind, atoms):
+        comps = sorted(set(int(comp_of[a]) for a in atoms))
+        for i in atoms:
+            for j in atoms:
+                if i == j:
+                    continue
+                e = (int(comp_of[j]), int(local_of[j]))
+                if e not in ignores[i]:
+                    ignores[i].append(e)
+        if len(comps) == 1:
+            c = comps[0]
+            loc = [int(local_of[a]) for a in atoms]
+            {"b": c_bonds, "a": c_angles, "d": c_dihs}[kind][c].append(loc)
+        else:
+            br = bridges.setdefault((comps[0], comps[1]), {"particles": [], "b": [], "a": [], "d": []})
+            for a in atoms:
+                if a not in br["particles"]:
+                    br["particles"].append(a)
+            br[kind].append([br["particles"].index(a) for a in atoms])
+
+    for ch in range(n_chains):
+        s = ch * n_beads_chain
+        e = s + n_beads_chain
+        for i in range(s, e - 1):
+            add_term("b", (i, i + 1))
+        for i in range(s, e - 2):
+            add_term("a", (i, i + 1, i + 2))
+        for i in range(s, e - 3):
+            add_term("d", (i, i + 1, i + 2, i + 3))
+
+    def pack(lists, kind):
+        off = np.zeros(len(lists) + 1, dtype=np.int32)
+        flat = []
+        for k, lst in enumerate(lists):
+            off[k + 1] = off[k] + len(lst)
+            flat.extend(lst)
+        if kind == "b":
+            arr = np.zeros(len(flat), dtype=PAIR_DTYPE)
+            arr["b0"], arr["kb"] = BOND_B0, BOND_KB
+        elif kind == "a":
+            arr = np.zeros(len(flat), dtype=ANGLE_DTYPE)
+            arr["theta_0"], arr["k_theta"] = ANGLE_THETA0, ANGLE_K
+        else:
+            arr = np.zeros(len(flat), dtype=DIHEDRAL_DTYPE)
+            arr["phi_0"], arr["k_phi"], arr["n"] = DIHEDRAL_PHI0, DIHEDRAL_K, DIHEDRAL_N
+        if flat:
+            arr["idx"] = np.array(flat, dtype=np.uint32)
+        return off, arr
+
+    bond_off, bonds = pack(c_bonds, "b")
+    angle_off, angles = pack(c_angles, "a")
+    dih_off, dihs = pack(c_dihs, "d")
+    br_keys = sorted(bridges)
+    br_list = [bridges[k] for k in br_keys]
+    bpo = np.zeros(len(br_list) + 1, dtype=np.int32)
+    bpc, bpl = [], []
+    for k, br in enumerate(br_list):
+        bpo[k + 1] = bpo[k] + len(br["particles"])
+        bpc.extend(int(comp_of[a]) for a in br["particles"])
+        bpl.extend(int(local_of[a]) for a in br["particles"])
+    bb_off, bb = pack([br["b"] for br in br_list], "b")
+    ba_off, ba = pack([br["a"] for br in br_list], "a")
+    bd_off, bd = pack([br["d"] for br in br_list], "d")
+    ig_off = np.zeros(n_chain + 1, dtype=np.int32)
+    for i in range(n_chain):
+        ig_off[i + 1] = ig_off[i] + len(ignores[i])
+    ig_c = np.array([e[0] for lst in ignores for e in lst], dtype=np.int32)
+    ig_l = np.array([e[1] for lst in ignores for e in lst], dtype=np.int32)
+
+    # ---- solvent lattice ----
+    nl = max(1, int(round(L * solvent_density ** (1.0 / 3.0))))
+    a = L / nl
+    g = (np.arange(nl) + 0.5) * a
+    sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    sites = sites + rng.uniform(-0.02, 0.02, size=sites.shape)
+    sites = np.mod(sites, L)
+    if n_chain > 0:
+        from scipy.spatial import cKDTree
+        wrapped = np.mod(cpos.astype(np.float64), L)
+        wrapped[wrapped >= L] = 0.0
+        tree = cKDTree(wrapped, boxsize=L)
+        sites_c = sites.copy()
+        sites_c[sites_c >= L] = 0.0
+        d, _ = tree.query(sites_c, k=1)
+        sites = sites[d >= SOLVENT_EXCLUSION]
+    if n_solvents is not None:
+        if n_solvents > len(sites):
+            raise ValueError(f"only {len(sites)} free lattice sites for {n_solvents} solvents")
+        pick = np.sort(rng.choice(len(sites), size=n_solvents, replace=False))
+        sites = sites[pick]
+    spos = sites.astype(np.float32)
+    ns = len(spos)
+
+    # ---- Maxwell velocities, prev = x - v dt ----
+    vc = rng.normal(size=(n_chain, 3)) * np.sqrt(GAS_R * T0 / masses)[:, None]
+    vs = rng.normal(size=(ns, 3)) * math.sqrt(GAS_R * T0 / float(ff[0, 0]))
+    cprev = (cpos.astype(np.float64) - vc * dt).astype(np.float32)
+    sprev = (spos.astype(np.float64) - vs * dt).astype(np.float32)
+
+    sys_ = LimaSystem(
+        box_len=L, dt=dt, thermostat_target=T0, forcefield=ff,
+        compound_offset=coff, key_particle_index=key, confining_particle_sphere=sphere,
+        chain_positions=cpos, chain_prev_positions=cprev, chain_atom_types=types,
+        bond_offset=bond_off, bonds=bonds, angle_offset=angle_off, angles=angles,
+        dihedral_offset=dih_off, dihedrals=dihs,
+        ignore_offset=ig_off, ignore_compound=ig_c, ignore_local=ig_l,
+        bridge_particle_offset=bpo, bridge_particle_compound=np.array(bpc, dtype=np.int32),
+        bridge_particle_local=np.array(bpl, dtype=np.int32),
+        bridge_bond_offset=bb_off, bridge_bonds=bb, bridge_angle_offset=ba_off, bridge_angles=ba,
+        bridge_dihedral_offset=bd_off, bridge_dihedrals=bd,
+        solvent_positions=spos, solvent_prev_positions=sprev,
+        meta={"seed": seed, "n_chains": n_chains, "compounds_per_chain": compounds_per_chain,
+              "n_types": n_types},
+    )
+    return sys_
+
+
+# ---------------------------------------------------------------------------
+# Named workloads.  seed = 0x51A7 + 1000*config + replica.
+# ---------------------------------------------------------------------------
+SEED_BASE = 0x51A7
+
+
+def small(seed_offset: int = 0) -> LimaSystem:
+    """1 chain of 256 (2 compounds x 128) + 3000 solvent, 5 nm, 8 types, 300 K."""
+    return build_system(SEED_BASE + seed_offset, 1, 2, 5.0, 8, 3000, T0=300.0)
+
+
+def production() -> LimaSystem:
+    """10 chains x 10 compounds x 128 (12,800 chain) + lattice solvent at 33/nm^3, 10 nm, 16 types."""
+    return build_system(SEED_BASE + 1000, 10, 10, 10.0, 16, None, T0=300.0)
+
+
+def large() -> LimaSystem:
+    """10 chains x 40 compounds x 128 (51,200 chain) + lattice solvent at 33/nm^3, 18 nm, 16 types."""
+    return build_system(SEED_BASE + 2000, 10, 40, 18.0, 16, None, T0=300.0)
+
+
+def replicas(n: int = 256, seed_config: int = 3) -> list[LimaSystem]:
+    """Uneven replicas: chain 128-1024 (1-8 compounds), solvent 500-6000, box from density 33 (3-6 nm)."""
+    out = []
+    for r in range(n):
+        seed = SEED_BASE + 1000 * seed_config + r
+        rng = np.random.default_rng(seed ^ 0x5EED)
+        ncomp = int(rng.integers(1, 9))
+        nsol = int(rng.integers(500, 6001))
+        T0 = float(rng.uniform(280.0, 320.0))
+        L = ((ncomp * COMPOUND_SIZE + nsol) / 33.0) ** (1.0 / 3.0)
+        L = float(min(6.0, max(3.0, L)))
+        cap = int(0.8 * 33.0 * L ** 3) - ncomp * COMPOUND_SIZE * 3
+        nsol = max(100, min(nsol, cap))
+        out.append(build_system(seed, 1, ncomp, L, 16, nsol, T0=T0))
+    return out
+
+
+def adjoint_replicas(n: int = 64) -> list[LimaSystem]:
+    """64 replicas of 1 chain of 512 + 1,488 solvent in a 4.2 nm box, 16 types (shared force field)."""
+    ff = make_forcefield(np.random.default_rng(SEED_BASE + 4000 - 1), 16)
+    return [build_system(SEED_BASE + 4000 + r, 1, 4, 4.2, 16, 1488, T0=300.0, forcefield=ff) for r in range(n)]
