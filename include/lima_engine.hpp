@@ -1,0 +1,83 @@
+// lima_engine.hpp — header-only C++ wrapper over lima.h with the reference's
+// Engine interface (Quantom/Quantom/Engine.cuh:109-208), for drop-in callers of
+// the per-step path:
+//
+//     lima::Engine engine(systems, params);      // Engine(Simulation*, ForceField), Engine.cu:9-38
+//     for (...) { engine.deviceMaster();         // Engine.cu:44-47  -> one step
+//                 engine.hostMaster(); }         // Engine.cu:49-75  -> error check (work is on the device)
+//     engine.timings;                            // Engine.cuh:123 (us: device, rebuild, host)
+//
+// Errors throw lima::Error carrying the lima.h status code instead of exit().
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lima.h"
+
+namespace lima {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Int3 {
+    long long x = 0, y = 0, z = 0;
+};
+
+inline lima_run_params default_params() {
+    lima_run_params p{};
+    p.cutoff = 0.6f;
+    p.nlist_interval = 10;
+    p.thermostat = LIMA_THERMOSTAT_BERENDSEN;
+    p.thermostat_tau = 1e-4;
+    p.accumulation = LIMA_ACCUM_FP32;
+    p.enable_solvents = 1;
+    p.enable_dihedrals = 1;
+    p.max_compound_neighbors = 256;
+    p.max_compound_solvents = 16384;
+    p.max_solvent_neighbors = 512;
+    p.device = 0;
+    return p;
+}
+
+class Engine {
+public:
+    Engine(const std::vector<lima_system>& systems, const lima_run_params& params) {
+        check(lima_create(systems.data(), (int32_t)systems.size(), &params, &h_), nullptr);
+    }
+    Engine(const lima_system& system, const lima_run_params& params) : Engine(std::vector<lima_system>{system}, params) {}
+    ~Engine() { lima_destroy(h_); }
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void deviceMaster() { check(lima_step(h_, 1)); }
+    void hostMaster() {
+        check(lima_sync(h_));
+        double t[5];
+        check(lima_get_timings(h_, t));
+        timings.x = (long long)t[0];
+        timings.y = (long long)t[1];
+        timings.z = (long long)t[2];
+    }
+    void step(int n) { check(lima_step(h_, n)); }
+    void rebuildNeighborLists() { check(lima_rebuild_neighbors(h_)); }
+    std::vector<lima_energies> energies(int n_systems) {
+        std::vector<lima_energies> e(n_systems);
+        check(lima_get_energies(h_, e.data()));
+        return e;
+    }
+    lima_engine* handle() { return h_; }
+
+    Int3 timings;
+
+private:
+    lima_engine* h_ = nullptr;
+    void check(int st) { check(st, h_); }
+    static void check(int st, lima_engine* h) {
+        if (st != LIMA_OK) throw Error(st, lima_last_error(h));
+    }
+};
+
+}  // namespace lima

@@ -40,7 +40,7 @@ struct ReplicaInfo {
     int cell_begin;               // first cell of this replica's solvent grid
     int ncell;                    // cells per dimension
     float cell_inv;               // ncell / L
-    int pad0;
+    float ss_bound;               // squared-distance bound of the solvent-solvent test (C + C)
 };
 
 // work item of the all-pairs solvent<-chain kernel
@@ -87,6 +87,16 @@ LIMA_DEV bool within(float ax, float ay, float az, float bx, float by, float bz,
     hyperpos(ax, ay, az, bx, by, bz, L, half);
     float d = exact_len(fsub(ax, bx), fsub(ay, by), fsub(az, bz));
     return d < fadd(cutoff, offset);
+}
+
+// the squared length inside Engine::neighborWithinCutoff: b re-imaged against a,
+// then (x*x + y*y) + z*z of a - b, exactly rounded.  Because sqrtf is correctly
+// rounded and monotone, sqrtf(d2) < t  <=>  d2 < B(t) with B(t) the smallest
+// float whose sqrtf is >= t (computed on the host), so the comparison is exact.
+LIMA_DEV float dist2_reimaged(float ax, float ay, float az, float bx, float by, float bz, float L, float half) {
+    hyperpos(ax, ay, az, bx, by, bz, L, half);
+    const float dx = fsub(ax, bx), dy = fsub(ay, by), dz = fsub(az, bz);
+    return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
 }
 
 // integratePosition (Engine.cu:849-869); returns the displacement after rescale.

@@ -112,9 +112,16 @@ struct lima_engine {
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
     DBuf<uint32_t> err;
-    DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell;
+    DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
+    DBuf<float4> ss_part, ss_img;
+    DBuf<float> comp_cs_bound;
 
-    hipStream_t stream = nullptr;
+    // the step is a fork/join DAG over three streams (all at default priority):
+    // stream: solvent<-chain all-pairs -> solvent integrate -> thermostat [-> rebuild]
+    // s1:     compound listed LJ -> compound integrate
+    // s2:     bridge, solvent listed LJ
+    hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
 
@@ -128,7 +135,18 @@ struct lima_engine {
 
     ~lima_engine() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c})
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t s : {stream, s1, s2})
+            if (s) (void)hipStreamDestroy(s);
+    }
+
+    void create_streams() {
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c})
+            HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
 
     StepArgs step_args(int parity) {
@@ -172,6 +190,7 @@ struct lima_engine {
         a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p;
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
+        a.ss_part = ss_part.p; a.ss_img = ss_img.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
@@ -205,6 +224,8 @@ struct lima_engine {
         a.n_cells = n_cells;
         a.cell_cnt = cell_cnt.p; a.cell_start = cell_start.p; a.cell_cursor = cell_cursor.p;
         a.cell_ids = cell_ids.p; a.solv_cell = solv_cell.p;
+        a.cell_rep = cell_rep.p;
+        a.comp_cs_bound = comp_cs_bound.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
@@ -212,24 +233,40 @@ struct lima_engine {
     }
 
     // ---- launches ----
+    template <bool FIXED, bool ENERGY> void launch_solvent_chain(const StepArgs& a, hipStream_t st) {
+        const int nw = (int)scwork_h.size();
+        switch (cfg.spt) {
+            case 1: solvent_chain_kernel<1, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
+            case 4: solvent_chain_kernel<4, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
+            default: solvent_chain_kernel<2, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
+        }
+    }
+
+    // one force pass (and integration when ADVANCE) as a fork/join over three streams
     template <bool ADVANCE, bool FIXED, bool ENERGY> void launch_forces(int parity) {
         StepArgs a = step_args(parity);
-        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, stream>>>(a);
-        if (params.enable_solvents && !scwork_h.empty()) {
-            const int nw = (int)scwork_h.size();
-            switch (cfg.spt) {
-                case 1: solvent_chain_kernel<1, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
-                case 4: solvent_chain_kernel<4, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
-                default: solvent_chain_kernel<2, FIXED, ENERGY><<<nw, 256, 0, stream>>>(a); break;
-            }
-        }
+        const bool solv = params.enable_solvents && NS > 0;
+        HIP_TRY(hipEventRecord(ev_fork, stream));
+        HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
+        HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
+        // s2: bridge carry-in forces, then the listed solvent-solvent LJ
+        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
+        HIP_TRY(hipEventRecord(ev_bridge, s2));
+        if (solv) solvent_listed_kernel<ENERGY><<<n_stiles, 256, 0, s2>>>(a);
+        HIP_TRY(hipEventRecord(ev_ss, s2));
+        // s1: compound listed LJ, then bonded + intra + integration of the compounds
         if (NC > 0) {
             const int grid = cfg.listed_grid > 0 ? cfg.listed_grid : 4096;
-            compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, stream>>>(a);
-            compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, stream>>>(a);
+            compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, s1>>>(a);
         }
-        if (params.enable_solvents && NS > 0)
-            solvent_integrate_kernel<ADVANCE, FIXED, ENERGY><<<n_stiles, 256, 0, stream>>>(a);
+        HIP_TRY(hipStreamWaitEvent(s1, ev_bridge, 0));
+        if (NC > 0) compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, s1>>>(a);
+        HIP_TRY(hipEventRecord(ev_c, s1));
+        // main stream: all-pairs solvent <- chain, then the solvent integrator
+        if (solv && !scwork_h.empty()) launch_solvent_chain<FIXED, ENERGY>(a, stream);
+        HIP_TRY(hipStreamWaitEvent(stream, ev_ss, 0));
+        if (solv) solvent_integrate_kernel<ADVANCE, FIXED, ENERGY><<<n_stiles, 256, 0, stream>>>(a);
+        HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
     }
 
     void launch_step(int parity) {
@@ -255,7 +292,7 @@ struct lima_engine {
             nl_cell_scan_kernel<<<1, 1024, 0, stream>>>(a);
             nl_cell_scatter_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
             nl_cell_sort_kernel<<<n_cells, 256, 0, stream>>>(a);
-            nl_solvent_solvent_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
+            nl_solvent_solvent_kernel<<<n_cells, 128, 0, stream>>>(a);
         }
         if (NC > 0) nl_work_kernel<<<1, 1024, 0, stream>>>(a);
     }
@@ -336,8 +373,27 @@ struct lima_engine {
 
 namespace {
 
+// Smallest non-negative float x with sqrtf(x) >= t.  sqrtf is correctly rounded
+// (here and on the device) and monotone, so  sqrtf(d2) < t  <=>  d2 < sqrt_bound(t):
+// the neighbour tests compare squared distances with the exact same outcome.
+float sqrt_bound(float t) {
+    uint32_t lo = 0, hi = 0x7f800000u;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float x;
+        std::memcpy(&x, &mid, 4);
+        if (std::sqrt(x) >= t) hi = mid;
+        else lo = mid + 1;
+    }
+    float r;
+    std::memcpy(&r, &lo, 4);
+    return r;
+}
+
 void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     const lima_run_params& P = E->params;
+    std::vector<float> cs_bound;
+    std::vector<int> cell_rep_h;
     if (nsys <= 0) throw LimaError(LIMA_E_ARG, "n_systems must be >= 1");
     if (!(P.cutoff > 0)) throw LimaError(LIMA_E_ARG, "cutoff must be > 0");
     if (P.max_compound_neighbors <= 0 || P.max_compound_solvents <= 0 || P.max_solvent_neighbors <= 0)
@@ -412,6 +468,8 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
             comp_rep.push_back(r);
             comp_key.push_back(s.key_particle_index[c]);
             comp_sphere.push_back(s.confining_particle_sphere[c]);
+            // threshold of the compound-solvent test, C + (r + C) (Engine.cu:182, 231)
+            cs_bound.push_back(sqrt_bound(P.cutoff + (s.confining_particle_sphere[c] + P.cutoff)));
             const int gbase = chb + o0;
             const int nbd = s.bond_offset[c + 1] - s.bond_offset[c];
             const int nan = s.angle_offset[c + 1] - s.angle_offset[c];
@@ -558,13 +616,15 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
         for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += 256)
             stl.push_back(make_int4(r, t0, std::min(t0 + 256, ri.solv_end), (int)stl.size()));
         ri.stile_end = (int)stl.size();
-        const float thr = P.cutoff + P.cutoff;
+        const float thr = P.cutoff + P.cutoff;   // Engine.cu:198, 264
+        ri.ss_bound = sqrt_bound(thr);
         int ncell = (int)std::floor(ri.L / (thr * (1.0f + 1e-5f)));
         ncell = std::max(1, std::min(ncell, 256));
         ri.ncell = ncell;
         ri.cell_inv = (float)ncell / ri.L;
         ri.cell_begin = cell_begin;
         cell_begin += ncell * ncell * ncell;
+        cell_rep_h.insert(cell_rep_h.end(), (size_t)ncell * ncell * ncell, r);
     }
     // heaviest work items first
     std::stable_sort(E->scwork_h.begin(), E->scwork_h.end(), [&](const SCWork& x, const SCWork& y) {
@@ -635,6 +695,10 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->err.alloc(4); E->err.zero();
     E->cell_cnt.alloc(E->n_cells); E->cell_start.alloc(E->n_cells + 1); E->cell_cursor.alloc(E->n_cells);
     E->cell_ids.alloc(NSx); E->solv_cell.alloc(NSx);
+    E->cell_rep.upload(cell_rep_h);
+    E->comp_cs_bound.upload(cs_bound);
+    E->ss_part.alloc(NSx); E->ss_part.zero();
+    E->ss_img.alloc(NSx); E->ss_img.zero();
 }
 
 thread_local std::string g_last_error;
@@ -682,7 +746,7 @@ int lima_create(const lima_system* systems, int32_t n_systems, const lima_run_pa
         if (params->device < 0 || params->device >= ndev) throw LimaError(LIMA_E_ARG, "device ordinal out of range");
         E->device = params->device;
         HIP_TRY(hipSetDevice(E->device));
-        HIP_TRY(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+        E->create_streams();
         validate_and_build(E, systems, n_systems);
         // Engine::Engine -> handleNLISTS(force) (Engine.cu:35)
         E->launch_rebuild(E->cur);
@@ -923,9 +987,10 @@ int lima_get_timings(lima_engine* E, double* out5) {
     });
 }
 
-int lima_time_kernels(lima_engine* E, int32_t reps, double* out6) {
+int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
     return guarded(E, [&] {
-        if (!out6 || reps <= 0) throw LimaError(LIMA_E_ARG, "bad arguments");
+        if (!out8 || reps <= 0) throw LimaError(LIMA_E_ARG, "bad arguments");
+        double* out6 = out8;
         HIP_TRY(hipSetDevice(E->device));
         HIP_TRY(hipStreamSynchronize(E->stream));
         hipEvent_t e0, e1;
@@ -943,21 +1008,16 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out6) {
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
             return ms * 1000.0 / reps;
         };
-        for (int k = 0; k < 6; k++) out6[k] = 0.0;
+        for (int k = 0; k < 8; k++) out8[k] = 0.0;
         const int nw = (int)E->scwork_h.size();
         if (E->params.enable_solvents && nw > 0) {
             out6[0] = timeit([&] {
-                if (fx) {
-                    if (E->cfg.spt == 1) solvent_chain_kernel<1, true, false><<<nw, 256, 0, E->stream>>>(a);
-                    else if (E->cfg.spt == 4) solvent_chain_kernel<4, true, false><<<nw, 256, 0, E->stream>>>(a);
-                    else solvent_chain_kernel<2, true, false><<<nw, 256, 0, E->stream>>>(a);
-                } else {
-                    if (E->cfg.spt == 1) solvent_chain_kernel<1, false, false><<<nw, 256, 0, E->stream>>>(a);
-                    else if (E->cfg.spt == 4) solvent_chain_kernel<4, false, false><<<nw, 256, 0, E->stream>>>(a);
-                    else solvent_chain_kernel<2, false, false><<<nw, 256, 0, E->stream>>>(a);
-                }
+                if (fx) E->launch_solvent_chain<true, false>(a, E->stream);
+                else E->launch_solvent_chain<false, false>(a, E->stream);
             });
         }
+        if (E->params.enable_solvents && E->NS > 0)
+            out8[6] = timeit([&] { solvent_listed_kernel<false><<<E->n_stiles, 256, 0, E->stream>>>(a); });
         if (E->NC > 0) {
             const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
             out6[1] = timeit([&] {

@@ -70,6 +70,8 @@ struct StepArgs {
     const int* cc_cnt; const int* cc_ids;
     const int* cs_cnt; const int* cs_ids;
     const int* ss_cnt; const int* ss_ids;   // transposed: ss_ids[k*n_solv + s]
+    float4* ss_part;              // solvent-solvent local sums
+    float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
     // partial forces
     float4* partC;  I3* partC_fx;   // [slot*128 + p]
@@ -540,14 +542,15 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// solventForceKernel remainder (Engine.cu:1121-1155): solvent-solvent LJ over
-// the solvent's list (the self position is re-imaged against every neighbour,
-// Engine.cu:710), integrate, PBC, NaN check.  One thread per solvent; tiles
-// never straddle replicas.
+// solventForceKernel's listed part (Engine.cu:1111, 1121-1123, 706-720): the
+// solvent's own position is re-imaged against every compound's particle 0 in
+// compound order, then against every listed neighbour in list order, as the
+// reference mutates it; the solvent-solvent LJ local sum and the final self
+// image are written for the integrator.  One thread per solvent; it runs
+// concurrently with the all-pairs kernel.
 // ---------------------------------------------------------------------------
-template <bool ADVANCE, bool FIXED, bool ENERGY>
-__global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
-    __shared__ double red[4];
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
     const int4 tile = a.stiles[blockIdx.x];
     const int r = tile.x;
     const ReplicaInfo rp = a.reps[r];
@@ -561,6 +564,57 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
         float4 q = a.cpos[a.comp_off[c]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
     }
+    const float2 pss = a.tab_ss[r];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    double en = 0, vir = 0;
+    const int cnt = active ? a.ss_cnt[si] : 0;
+    const int* ids = a.ss_ids + si;
+    const size_t ld = (size_t)a.n_solv;
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const int i0 = ids[(k + 0) * ld], i1 = ids[(k + 1) * ld], i2 = ids[(k + 2) * ld], i3 = ids[(k + 3) * ld];
+        const float4 q[4] = {a.spos[i0], a.spos[i1], a.spos[i2], a.spos[i3]};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
+            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
+            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
+        }
+    }
+    for (; k < cnt; k++) {
+        const float4 q = a.spos[ids[k * ld]];
+        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
+        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
+        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
+    }
+    if (active) {
+        a.ss_part[s] = make_float4(lx, ly, lz, 0.f);
+        a.ss_img[s] = make_float4(sx, sy, sz, 0.f);
+    }
+    if (ENERGY) {
+        block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_ss);
+        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solventForceKernel remainder (Engine.cu:1129-1155): force = compound
+// contributions (chunk partials in order) + the solvent-solvent local sum;
+// integrate, PBC, NaN check.  Tiles never straddle replicas.
+// ---------------------------------------------------------------------------
+template <bool ADVANCE, bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
+    __shared__ double red[4];
+    const int4 tile = a.stiles[blockIdx.x];
+    const int r = tile.x;
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half;
+    const int s = tile.y + threadIdx.x;
+    const bool active = s < tile.z;
+    const int si = active ? s : tile.y;
+    const float4 me = a.spos[si];
+    const float4 img = a.ss_img[si];
+    float sx = img.x, sy = img.y, sz = img.z;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     long long ix = 0, iy = 0, iz = 0;
     for (int k = 0; k < rp.chunk_count; k++) {
@@ -568,21 +622,12 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
         if (FIXED) { I3 v = a.partS_fx[slot]; ix += v.x; iy += v.y; iz += v.z; }
         else { float4 v = a.partS[slot]; fx += v.x; fy += v.y; fz += v.z; }
     }
-    const float2 pss = a.tab_ss[r];
-    float lx = 0.f, ly = 0.f, lz = 0.f;
-    double en = 0, vir = 0;
-    const int cnt = active ? a.ss_cnt[si] : 0;
-    for (int k = 0; k < cnt; k++) {
-        const float4 q = a.spos[a.ss_ids[(size_t)k * a.n_solv + si]];
-        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
-        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
-    }
+    const float4 l = a.ss_part[si];
     if (FIXED) {
-        ix += to_fixed(lx); iy += to_fixed(ly); iz += to_fixed(lz);
+        ix += to_fixed(l.x); iy += to_fixed(l.y); iz += to_fixed(l.z);
         fx = from_fixed(ix); fy = from_fixed(iy); fz = from_fixed(iz);
     } else {
-        fx += lx; fy += ly; fz += lz;
+        fx += l.x; fy += l.y; fz += l.z;
     }
     if (active) a.sforce[s] = make_float4(fx, fy, fz, 0.f);
     const double m = (double)a.mass[r * kMaxTypes + 0];
@@ -613,10 +658,6 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
         for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
         if (ADVANCE) a.ke_stile[tile.w] = tot;
         if (ENERGY) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
-    }
-    if (ENERGY) {
-        block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_ss);
-        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
     }
 }
 
@@ -678,6 +719,8 @@ struct NlistArgs {
     int* ss_cnt; int* ss_ids;
     int n_cells;
     int* cell_cnt; int* cell_start; int* cell_cursor; int* cell_ids; int* solv_cell;
+    const int* cell_rep;          // replica of each cell
+    const float* comp_cs_bound;   // per compound: bound(C + (r + C)) for the squared distance
     int* cw_off; int* cw_total;
     uint32_t* err;
     int enable_solvents;
@@ -741,7 +784,9 @@ __global__ __launch_bounds__(256) void nl_compound_solvent_kernel(NlistArgs a) {
     const ReplicaInfo rp = a.reps[r];
     const float L = rp.L, half = rp.half, C = a.cutoff;
     const float4 ki = key_pos(a, i);
-    const float off = fadd(a.comp_sphere[i], C);
+    // |d| < C + (r + C)  <=>  d^2 < bound (correctly rounded sqrt is monotone)
+    const float bound = a.comp_cs_bound[i];
+    (void)C;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int count = 0;
     for (int s0 = rp.solv_begin; s0 < rp.solv_end; s0 += 256) {
@@ -749,7 +794,7 @@ __global__ __launch_bounds__(256) void nl_compound_solvent_kernel(NlistArgs a) {
         bool member = false;
         if (s < rp.solv_end) {
             const float4 q = a.spos[s];
-            member = within(ki.x, ki.y, ki.z, q.x, q.y, q.z, off, C, L, half);
+            member = dist2_reimaged(ki.x, ki.y, ki.z, q.x, q.y, q.z, L, half) < bound;
         }
         const unsigned long long ball = __ballot(member);
         if (lane == 0) wave_cnt[wid] = __popcll(ball);
@@ -844,43 +889,84 @@ __global__ __launch_bounds__(256) void nl_cell_sort_kernel(NlistArgs a) {
     }
 }
 
-// one thread per solvent: scan the (deduplicated) 27 neighbouring cells
-__global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n_solv) return;
-    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
-    const float L = rp.L, half = rp.half, C = a.cutoff;
-    const float4 me = a.spos[s];
+// One block per home cell: the solvents of the (deduplicated) 27 neighbouring
+// cells are staged in LDS in a fixed cell order (each cell sorted by id); every
+// thread tests one home solvent against all candidates.  The pair predicate is
+// evaluated with the lower id as the static point, as the reference's add loop
+// does (Engine.cu:262-264), and compared as d^2 < bound(C + C).
+constexpr int kCellCandidates = 4096;
+
+LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, float bound) {
+    const bool lo_self = s < j;
+    const float ax = lo_self ? me.x : q.x, ay = lo_self ? me.y : q.y, az = lo_self ? me.z : q.z;
+    const float bx = lo_self ? q.x : me.x, by = lo_self ? q.y : me.y, bz = lo_self ? q.z : me.z;
+    return (j != s) && dist2_reimaged(ax, ay, az, bx, by, bz, L, half) < bound;
+}
+
+__global__ __launch_bounds__(128) void nl_solvent_solvent_kernel(NlistArgs a) {
+    __shared__ float4 cand_pos[kCellCandidates];
+    __shared__ int cand_id[kCellCandidates];
+    const int cell = blockIdx.x;
+    const int h0 = a.cell_start[cell], h1 = a.cell_start[cell + 1];
+    if (h0 == h1) return;
+    const ReplicaInfo rp = a.reps[a.cell_rep[cell]];
+    const float L = rp.L, half = rp.half, bound = rp.ss_bound;
     const int n = rp.ncell;
-    const int cx = cell_coord(me.x, rp.cell_inv, n), cy = cell_coord(me.y, rp.cell_inv, n), cz = cell_coord(me.z, rp.cell_inv, n);
+    const int local = cell - rp.cell_begin;
+    const int cx = local / (n * n), cy = (local / n) % n, cz = local % n;
     const int span = n >= 3 ? 3 : n;
-    const int lo_off = n >= 3 ? -1 : 0;
-    int count = 0;
+    int nb[27];
+    int nnb = 0;
     for (int ox = 0; ox < span; ox++) {
-        const int x = (n >= 3) ? (cx + lo_off + ox + n) % n : ox;
+        const int x = (n >= 3) ? (cx - 1 + ox + n) % n : ox;
         for (int oy = 0; oy < span; oy++) {
-            const int y = (n >= 3) ? (cy + lo_off + oy + n) % n : oy;
+            const int y = (n >= 3) ? (cy - 1 + oy + n) % n : oy;
             for (int oz = 0; oz < span; oz++) {
-                const int z = (n >= 3) ? (cz + lo_off + oz + n) % n : oz;
-                const int cell = rp.cell_begin + (x * n + y) * n + z;
-                const int e1 = a.cell_start[cell + 1];
-                for (int e = a.cell_start[cell]; e < e1; e++) {
+                const int z = (n >= 3) ? (cz - 1 + oz + n) % n : oz;
+                nb[nnb++] = rp.cell_begin + (x * n + y) * n + z;
+            }
+        }
+    }
+    int total = 0;
+    for (int k = 0; k < nnb; k++) {
+        const int b = a.cell_start[nb[k]], e = a.cell_start[nb[k] + 1];
+        for (int j = threadIdx.x; j < e - b; j += blockDim.x) {
+            if (total + j < kCellCandidates) {
+                const int id = a.cell_ids[b + j];
+                cand_id[total + j] = id;
+                cand_pos[total + j] = a.spos[id];
+            }
+        }
+        total += e - b;
+    }
+    __syncthreads();
+    for (int h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
+        const int s = a.cell_ids[h];
+        const float4 me = a.spos[s];
+        int count = 0;
+        if (total <= kCellCandidates) {
+            for (int e = 0; e < total; e++) {
+                const int j = cand_id[e];
+                if (ss_member(s, me, j, cand_pos[e], L, half, bound)) {
+                    if (count < a.cap_ss) a.ss_ids[(size_t)count * a.n_solv + s] = j;
+                    count++;
+                }
+            }
+        } else {   // more candidates than LDS holds: same order, read from global memory
+            for (int k = 0; k < nnb; k++) {
+                const int e1 = a.cell_start[nb[k] + 1];
+                for (int e = a.cell_start[nb[k]]; e < e1; e++) {
                     const int j = a.cell_ids[e];
-                    if (j == s) continue;
-                    const float4 q = a.spos[j];
-                    bool member;
-                    if (s < j) member = within(me.x, me.y, me.z, q.x, q.y, q.z, C, C, L, half);
-                    else member = within(q.x, q.y, q.z, me.x, me.y, me.z, C, C, L, half);
-                    if (member) {
+                    if (ss_member(s, me, j, a.spos[j], L, half, bound)) {
                         if (count < a.cap_ss) a.ss_ids[(size_t)count * a.n_solv + s] = j;
                         count++;
                     }
                 }
             }
         }
+        if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
+        a.ss_cnt[s] = min(count, a.cap_ss);
     }
-    if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
-    a.ss_cnt[s] = min(count, a.cap_ss);
 }
 
 // per compound work slots: listed neighbours + ceil(solvent list / 256); exclusive scan

@@ -228,11 +228,12 @@ class Engine:
         self._check(self.L.lima_get_timings(self.h, t))
         return {"device_us": t[0], "rebuild_us": t[1], "host_us": t[2], "steps": int(t[3]), "rebuilds": int(t[4])}
 
-    KERNELS = ("solvent_chain", "compound_listed", "compound_integrate", "solvent_integrate", "bridge", "rebuild")
+    KERNELS = ("solvent_chain", "compound_listed", "compound_integrate", "solvent_integrate", "bridge", "rebuild",
+               "solvent_listed", "reserved")
 
     def time_kernels(self, reps: int = 20) -> dict:
         """Average microseconds per launch of each step kernel, launched alone (HIP events)."""
-        t = (C.c_double * 6)()
+        t = (C.c_double * 8)()
         self._check(self.L.lima_time_kernels(self.h, reps, t))
         return dict(zip(self.KERNELS, list(t)))
 
