@@ -68,8 +68,92 @@ def cpu_baseline(systems_, params, budget_s):
                       f"oracle/liblima_oracle_fast.so (g++ -O3 -fopenmp, FMA allowed)"}
 
 
+def sequential_baseline(syss, params, steps):
+    """The same replicas stepped one engine at a time (plain single-system launches)."""
+    from quantom_amd.engine import Engine
+    engines = [Engine([s], params) for s in syss]
+    for e in engines:
+        e.step(params.nlist_interval)
+    t0 = time.perf_counter()
+    for e in engines:
+        e.step(steps)
+    t = time.perf_counter() - t0
+    for e in engines:
+        e.close()
+    return 1000.0 * t / steps
+
+
+def accuracy_report(args):
+    """rms force error vs FP64 (FP32 and int64 accumulation) on the production system, and
+    energy drift of a short NVE run on the small system beside the FP64 oracle's drift."""
+    import numpy as np
+
+    from oracle.oracle import KIND_CC, KIND_CS, KIND_SC, KIND_SS, Oracle
+    from quantom_amd import systems
+    from quantom_amd.abi import ACCUM_FIXED64, ACCUM_FP32, THERMOSTAT_NONE, RunParams
+    from quantom_amd.engine import Engine
+
+    def rel_rms(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+    out = {"mode": "accuracy"}
+    prod = systems.production()
+    p = RunParams(nlist_interval=0)
+    o64 = Oracle(prod, p, precision=64, fast=True)
+    o32 = Oracle(prod, p, precision=32, fast=True)
+    for k in (KIND_CC, KIND_CS, KIND_SC, KIND_SS):
+        o64.set_neighbors(k, o32.neighbors(k))
+    f64c, f64s, _ = o64.compute_forces()
+    ref = np.concatenate([f64c, f64s])
+    for name, acc in (("fp32", ACCUM_FP32), ("fixed64", ACCUM_FIXED64)):
+        e = Engine([prod], RunParams(nlist_interval=0, accumulation=acc))
+        e.set_neighbors(cc=o32.neighbors(KIND_CC), cs=o32.neighbors(KIND_CS), ss=o32.neighbors(KIND_SS))
+        cf, sf = e.compute_forces()
+        out[f"production_rms_force_error_{name}"] = rel_rms(np.concatenate([cf, sf]), ref)
+        e.close()
+    log(f"force errors: {out}")
+    # energy drift: NVE, small system, GPU (fp32, fixed64) and the FP64 oracle
+    small = systems.small()
+    steps, every = args.drift_steps, 50
+    pn = RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE)
+    ns = small.dt * 1e-0
+    series = {}
+    for name, acc in (("fp32", ACCUM_FP32), ("fixed64", ACCUM_FIXED64)):
+        e = Engine([small], RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE, accumulation=acc))
+        es = []
+        for k in range(0, steps + 1, every):
+            en = e.energies()[0]
+            es.append(en["potential"] + en["kinetic"])
+            if k < steps:
+                e.step(every)
+        series[name] = es
+        e.close()
+    o = Oracle(small, pn, precision=64, fast=True)
+    es = []
+    for k in range(0, steps + 1, every):
+        en = o.energies()
+        es.append(en["potential"] + en["kinetic"])
+        if k < steps:
+            o.step(every)
+    series["fp64_oracle"] = es
+    t_ns = np.arange(len(es)) * every * small.dt
+    n_part = small.n_chain + small.n_solvents
+    for name, s in series.items():
+        slope = float(np.polyfit(t_ns, np.asarray(s) / n_part, 1)[0])
+        out[f"small_energy_drift_{name}_J_per_mol_per_particle_per_ns"] = slope
+    out["drift_window_steps"] = steps
+    out["note"] = ("angle and dihedral forces of the reference are not gradients of their energies, "
+                   "so the drift is compared with the FP64 oracle's own drift")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", default="bench", choices=["bench", "accuracy"])
+    ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--sequential", action="store_true",
+                    help="replicas workload: also time the replicas as sequential single-system engines")
+    ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
@@ -87,6 +171,10 @@ def main():
     from quantom_amd.abi import ACCUM_FIXED64, ACCUM_FP32, RunParams
     from quantom_amd.engine import Engine
 
+    if args.mode == "accuracy":
+        print(json.dumps(accuracy_report(args)), flush=True)
+        return
+
     t_gen = time.perf_counter()
     syss, desc = workload(args.workload)
     log(f"generated {args.workload} in {time.perf_counter() - t_gen:.1f}s")
@@ -94,6 +182,10 @@ def main():
     n_solv = sum(s.n_solvents for s in syss)
     params = RunParams(nlist_interval=10, accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
     eng = Engine(syss, params)
+    if args.autotune:
+        t_at = time.perf_counter()
+        eng.autotune(10)
+        log(f"autotune {time.perf_counter() - t_at:.1f}s")
     log(f"engine ready: chain {n_chain} solvent {n_solv} config {eng.config()}")
     eng.step(args.warmup)
     eng.sync()
@@ -128,6 +220,15 @@ def main():
         "roofline": roof,
         "kernels_us": kt,
     }
+    if len(syss) > 1:
+        out["replica_ns_per_day_each"] = ns_day
+        out["value"] = ns_day * len(syss)          # whole job: simulated ns of all replicas per day
+        out["unit"] = "replica-ns/day"
+        if args.sequential:
+            ms_seq = sequential_baseline(syss, params, max(10, args.steps // 10))
+            out["sequential_ms_per_step"] = ms_seq
+            out["batched_over_sequential"] = ms_seq / ms
+            log(f"sequential single-system engines: {ms_seq:.3f} ms per step of all replicas")
     if not args.no_cpu and len(syss) == 1:
         cb = cpu_baseline(syss, params, args.cpu_budget)
         out["cpu_baseline"] = cb

@@ -12,7 +12,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -78,8 +83,10 @@ struct lima_engine {
     std::vector<int> sys_chain_begin, sys_solv_begin, sys_comp_begin;  // per system, into the concatenation
     int n_cells = 0, n_stiles = 0;
     int kmax = 0;                 // max solvent<-chain chunks in any replica
+    int chunk_cap = 1;            // chain particles of the largest chunk (LDS tile of the all-pairs kernel)
     size_t max_slots = 0;
     std::vector<SCWork> scwork_h;
+    std::vector<int> comp_off_h;
     int device = 0;
 
     // ---- device buffers ----
@@ -103,8 +110,9 @@ struct lima_engine {
     DBuf<int> solv_rep;
     DBuf<int4> stiles;
     DBuf<float> mass;
-    DBuf<float2> tab_sc, tab_cs, tab_inter, tab_intra, tab_ss;
-    DBuf<int> cc_cnt, cc_ids, cc_old_cnt, cc_old_ids, cs_cnt, cs_ids, ss_cnt, ss_ids;
+    DBuf<float4> tab_sc, tab_cs, tab_inter, tab_intra, tab_ss;
+    DBuf<int> cc_cnt, cc_ids, cs_cnt, cs_ids, ss_cnt, ss_ids, seg_cnt;
+    int max_segs = 1;
     DBuf<int> cw_off, cw_total;
     DBuf<float4> partC, partS;
     DBuf<I3> partC_fx, partS_fx;
@@ -195,6 +203,7 @@ struct lima_engine {
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
         a.scwork = scwork.p; a.n_scwork = (int)scwork_h.size();
+        a.chunk_cap = chunk_cap;
         a.lambda = lambda.p;
         a.ke_comp = ke_comp.p;
         a.ke_stile = ke_stile.p;
@@ -218,7 +227,8 @@ struct lima_engine {
         a.cap_cc = params.max_compound_neighbors;
         a.cap_cs = params.max_compound_solvents;
         a.cap_ss = params.max_solvent_neighbors;
-        a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p; a.cc_old_cnt = cc_old_cnt.p; a.cc_old_ids = cc_old_ids.p;
+        a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p;
+        a.seg_cnt = seg_cnt.p; a.max_segs = max_segs;
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
         a.n_cells = n_cells;
@@ -233,13 +243,63 @@ struct lima_engine {
     }
 
     // ---- launches ----
+    size_t chunk_lds_bytes() const { return (size_t)chunk_cap * (sizeof(float4) + sizeof(float2)); }
+
     template <bool FIXED, bool ENERGY> void launch_solvent_chain(const StepArgs& a, hipStream_t st) {
         const int nw = (int)scwork_h.size();
+        const size_t lds = chunk_lds_bytes();
         switch (cfg.spt) {
-            case 1: solvent_chain_kernel<1, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
-            case 4: solvent_chain_kernel<4, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
-            default: solvent_chain_kernel<2, FIXED, ENERGY><<<nw, 256, 0, st>>>(a); break;
+            case 1: solvent_chain_kernel<1, FIXED, ENERGY><<<nw, 256, lds, st>>>(a); break;
+            case 4: solvent_chain_kernel<4, FIXED, ENERGY><<<nw, 256, lds, st>>>(a); break;
+            default: solvent_chain_kernel<2, FIXED, ENERGY><<<nw, 256, lds, st>>>(a); break;
         }
+    }
+
+    // Work list of the all-pairs kernel: per replica, solvent tiles of 256*spt x
+    // chunks of consecutive compounds holding ~chunk_particles chain particles;
+    // heaviest items first.  Sizes the partial buffer and the LDS tile.
+    void build_scwork() {
+        const int tile_sz = 256 * cfg.spt;
+        scwork_h.clear();
+        kmax = 0;
+        chunk_cap = 1;
+        for (int r = 0; r < R; r++) {
+            ReplicaInfo& ri = reps_h[r];
+            std::vector<std::pair<int, int>> chunks;
+            int c = ri.comp_begin;
+            while (c < ri.comp_end) {
+                int c1 = c, np = 0;
+                while (c1 < ri.comp_end && (c1 - c) < kChunkCompounds) {
+                    const int n = comp_off_h[c1 + 1] - comp_off_h[c1];
+                    if (np > 0 && (np + n > cfg.chunk_particles || np + n > kChunkParticles)) break;
+                    np += n;
+                    c1++;
+                }
+                chunk_cap = std::max(chunk_cap, np);
+                chunks.push_back({c, c1});
+                c = c1;
+            }
+            if (ri.solv_end == ri.solv_begin) chunks.clear();
+            ri.chunk_count = (int)chunks.size();
+            kmax = std::max(kmax, ri.chunk_count);
+            for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += tile_sz)
+                for (int k = 0; k < (int)chunks.size(); k++)
+                    scwork_h.push_back(SCWork{r, t0, std::min(t0 + tile_sz, ri.solv_end), chunks[k].first, chunks[k].second, k, 0, 0});
+        }
+        std::stable_sort(scwork_h.begin(), scwork_h.end(), [&](const SCWork& x, const SCWork& y) {
+            const long long wx = (long long)(x.solv_end - x.solv_begin) * (comp_off_h[x.comp_end] - comp_off_h[x.comp_begin]);
+            const long long wy = (long long)(y.solv_end - y.solv_begin) * (comp_off_h[y.comp_end] - comp_off_h[y.comp_begin]);
+            return wx > wy;
+        });
+        const size_t need = std::max<size_t>(1, (size_t)kmax * std::max(NS, 1));
+        if (params.accumulation == LIMA_ACCUM_FIXED64) {
+            if (partS_fx.n < need) partS_fx.alloc(need);
+            if (!partS.p) partS.alloc(1);
+        } else {
+            if (partS.n < need) partS.alloc(need);
+            if (!partS_fx.p) partS_fx.alloc(1);
+        }
+        scwork.upload(scwork_h);
     }
 
     // one force pass (and integration when ADVANCE) as a fork/join over three streams
@@ -279,21 +339,29 @@ struct lima_engine {
                                                 temperature.p, params.thermostat == LIMA_THERMOSTAT_BERENDSEN ? 1 : 0);
     }
 
+    // the rebuild forks too: compound lists on s1, the solvent cell pipeline on the main stream
     void launch_rebuild(int parity) {
         NlistArgs a = nlist_args(parity);
+        const bool solv = params.enable_solvents && NS > 0;
+        HIP_TRY(hipEventRecord(ev_fork, stream));
+        HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
         if (NC > 0) {
-            nl_copy_cc_kernel<<<NC, 64, 0, stream>>>(a);
-            nl_compound_compound_kernel<<<NC, 64, 0, stream>>>(a);
-            if (params.enable_solvents && NS > 0) nl_compound_solvent_kernel<<<NC, 256, 0, stream>>>(a);
+            nl_compound_compound_kernel<<<NC, 64, 0, s1>>>(a);
+            if (solv) {
+                nl_cs_count_kernel<<<dim3(NC, max_segs), 256, 0, s1>>>(a);
+                nl_cs_write_kernel<<<dim3(NC, max_segs), 256, 0, s1>>>(a);
+            }
         }
-        if (params.enable_solvents && NS > 0) {
+        HIP_TRY(hipEventRecord(ev_c, s1));
+        if (solv) {
             nl_cell_zero_kernel<<<(n_cells + 255) / 256, 256, 0, stream>>>(cell_cnt.p, n_cells);
             nl_cell_count_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
             nl_cell_scan_kernel<<<1, 1024, 0, stream>>>(a);
             nl_cell_scatter_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
             nl_cell_sort_kernel<<<n_cells, 256, 0, stream>>>(a);
-            nl_solvent_solvent_kernel<<<n_cells, 128, 0, stream>>>(a);
+            nl_solvent_solvent_kernel<<<n_cells, 256, 0, stream>>>(a);
         }
+        HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
         if (NC > 0) nl_work_kernel<<<1, 1024, 0, stream>>>(a);
     }
 
@@ -390,6 +458,13 @@ float sqrt_bound(float t) {
     return r;
 }
 
+// LJ parameters of one call site: sigma^2 and 24*eps rounded as the reference
+// rounds sigma*sigma and 24.f*epsilon (Engine.cu:488-490), and -2*(24 eps).
+float4 pair_params(float sigma, float eps) {
+    const float e24 = 24.f * eps;
+    return make_float4(sigma * sigma, e24, -2.0f * e24, 0.0f);
+}
+
 void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     const lima_run_params& P = E->params;
     std::vector<float> cs_bound;
@@ -415,7 +490,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     std::vector<DihD> brd;
     std::vector<int> solv_rep;
     std::vector<float> mass(nsys * kMaxTypes);
-    std::vector<float2> tsc(nsys * kMaxTypes), tcs(nsys * kMaxTypes), tinter(nsys * 1024), tintra(nsys * 1024), tss(nsys);
+    std::vector<float4> tsc(nsys * kMaxTypes), tcs(nsys * kMaxTypes), tinter(nsys * 1024), tintra(nsys * 1024), tss(nsys);
     E->reps_h.clear();
     E->sys_chain_begin.clear(); E->sys_solv_begin.clear(); E->sys_comp_begin.clear();
     std::vector<int> bridge_owner;
@@ -446,18 +521,18 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
             mass[r * kMaxTypes + t] = pp[t].mass;
             float sg = (pp[0].sigma + pp[t].sigma) * 0.5f;
             float ep = (pp[0].epsilon + pp[t].epsilon) * 0.5f;
-            tsc[r * kMaxTypes + t] = make_float2(sg * sg, 24.f * ep);
+            tsc[r * kMaxTypes + t] = pair_params(sg, ep);
             float epg = sqrtf(pp[t].epsilon * pp[0].epsilon);
-            tcs[r * kMaxTypes + t] = make_float2(sg * sg, 24.f * epg);
+            tcs[r * kMaxTypes + t] = pair_params(sg, epg);
             for (int u = 0; u < kMaxTypes; u++) {
                 float su = (pp[t].sigma + pp[u].sigma) * 0.5f;
                 float ea = (pp[t].epsilon + pp[u].epsilon) * 0.5f;
                 float eg = sqrtf(pp[t].epsilon * pp[u].epsilon);
-                tinter[r * 1024 + t * 32 + u] = make_float2(su * su, 24.f * ea);
-                tintra[r * 1024 + t * 32 + u] = make_float2(su * su, 24.f * eg);
+                tinter[r * 1024 + t * 32 + u] = pair_params(su, ea);
+                tintra[r * 1024 + t * 32 + u] = pair_params(su, eg);
             }
         }
-        tss[r] = make_float2(pp[0].sigma * pp[0].sigma, 24.f * pp[0].epsilon);
+        tss[r] = pair_params(pp[0].sigma, pp[0].epsilon);
         // compounds
         for (int c = 0; c < s.n_compounds; c++) {
             const int o0 = s.compound_offset[c], o1 = s.compound_offset[c + 1];
@@ -582,36 +657,12 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->NS = (int)spos.size();
     E->NB = (int)br_rep.size();
 
-    // ---- solvent<-chain work list, solvent tiles, cell grids ----
-    const int spt = E->cfg.spt;
-    const int tile_sz = 256 * spt;
-    E->scwork_h.clear();
+    // ---- solvent tiles, cell grids (the solvent<-chain work list: build_scwork) ----
+    E->comp_off_h = comp_off;
     std::vector<int4> stl;
-    E->kmax = 0;
     int cell_begin = 0;
     for (int r = 0; r < nsys; r++) {
         ReplicaInfo& ri = E->reps_h[r];
-        // compound chunks: consecutive compounds up to the target particle count
-        std::vector<std::pair<int, int>> chunks;
-        int c = ri.comp_begin;
-        while (c < ri.comp_end) {
-            int c1 = c, np = 0;
-            while (c1 < ri.comp_end && (c1 - c) < kChunkCompounds) {
-                int n = comp_off[c1 + 1] - comp_off[c1];
-                if (np > 0 && (np + n > E->cfg.chunk_particles || np + n > kChunkParticles)) break;
-                np += n;
-                c1++;
-            }
-            chunks.push_back({c, c1});
-            c = c1;
-        }
-        const int ns_r = ri.solv_end - ri.solv_begin;
-        if (ns_r == 0) chunks.clear();
-        ri.chunk_count = (int)chunks.size();
-        E->kmax = std::max(E->kmax, ri.chunk_count);
-        for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += tile_sz)
-            for (int k = 0; k < (int)chunks.size(); k++)
-                E->scwork_h.push_back(SCWork{r, t0, std::min(t0 + tile_sz, ri.solv_end), chunks[k].first, chunks[k].second, k, 0, 0});
         ri.stile_begin = (int)stl.size();
         for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += 256)
             stl.push_back(make_int4(r, t0, std::min(t0 + 256, ri.solv_end), (int)stl.size()));
@@ -626,12 +677,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
         cell_begin += ncell * ncell * ncell;
         cell_rep_h.insert(cell_rep_h.end(), (size_t)ncell * ncell * ncell, r);
     }
-    // heaviest work items first
-    std::stable_sort(E->scwork_h.begin(), E->scwork_h.end(), [&](const SCWork& x, const SCWork& y) {
-        long long wx = (long long)(x.solv_end - x.solv_begin) * (comp_off[x.comp_end] - comp_off[x.comp_begin]);
-        long long wy = (long long)(y.solv_end - y.solv_begin) * (comp_off[y.comp_end] - comp_off[y.comp_begin]);
-        return wx > wy;
-    });
+    E->build_scwork();
     E->n_cells = std::max(cell_begin, 1);
     E->n_stiles = (int)stl.size();
 
@@ -665,10 +711,12 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->tab_sc.upload(tsc); E->tab_cs.upload(tcs); E->tab_inter.upload(tinter); E->tab_intra.upload(tintra); E->tab_ss.upload(tss);
     const size_t NCx = std::max(E->NC, 1), NSx = std::max(E->NS, 1);
     E->cc_cnt.alloc(NCx); E->cc_cnt.zero();
-    E->cc_old_cnt.alloc(NCx); E->cc_old_cnt.zero();
     E->cc_ids.alloc(NCx * P.max_compound_neighbors);
-    E->cc_old_ids.alloc(NCx * P.max_compound_neighbors);
     E->cs_cnt.alloc(NCx); E->cs_cnt.zero();
+    E->max_segs = 1;
+    for (const ReplicaInfo& ri : E->reps_h)
+        E->max_segs = std::max(E->max_segs, (ri.solv_end - ri.solv_begin + kSegSolvents - 1) / kSegSolvents);
+    E->seg_cnt.alloc(NCx * E->max_segs); E->seg_cnt.zero();
     E->cs_ids.alloc(NCx * P.max_compound_solvents);
     E->ss_cnt.alloc(NSx); E->ss_cnt.zero();
     E->ss_ids.alloc(NSx * P.max_solvent_neighbors);
@@ -677,14 +725,11 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
     if (P.accumulation == LIMA_ACCUM_FIXED64) {
         E->partC_fx.alloc(E->max_slots * kCompoundBlock);
-        E->partS_fx.alloc(std::max<size_t>(1, (size_t)E->kmax * NSx));
-        E->partC.alloc(1); E->partS.alloc(1);
+        E->partC.alloc(1);
     } else {
         E->partC.alloc(E->max_slots * kCompoundBlock);
-        E->partS.alloc(std::max<size_t>(1, (size_t)E->kmax * NSx));
-        E->partC_fx.alloc(1); E->partS_fx.alloc(1);
+        E->partC_fx.alloc(1);
     }
-    E->scwork.upload(E->scwork_h);
     std::vector<float> ones(nsys, 1.0f);
     E->lambda.upload(ones);
     E->lambda_applied.upload(ones);
@@ -716,6 +761,39 @@ template <typename F> int guarded(lima_engine* E, F&& f) {
     } catch (const LimaError& e) {
         return fail(E, e);
     }
+}
+
+// ---- autotune cache, keyed by the histogram of replica shapes ----
+std::mutex g_tune_mu;
+std::map<std::string, LaunchConfig> g_tune_cache;
+bool g_tune_loaded = false;
+
+std::string shape_key(const lima_engine* E) {
+    std::map<std::pair<int, int>, int> hist;   // (chain particles / 128, solvents / 512) -> replicas
+    for (const ReplicaInfo& ri : E->reps_h)
+        hist[{(ri.chain_end - ri.chain_begin + 64) / 128, (ri.solv_end - ri.solv_begin + 256) / 512}]++;
+    std::ostringstream os;
+    os << "acc" << E->params.accumulation << "_sol" << E->params.enable_solvents;
+    for (const auto& kv : hist) os << "_" << kv.first.first << "x" << kv.first.second << ":" << kv.second;
+    return os.str();
+}
+
+void load_tune_file() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char* path = std::getenv("LIMA_AUTOTUNE_CACHE");
+    if (!path) return;
+    std::ifstream f(path);
+    std::string key;
+    LaunchConfig c;
+    while (f >> key >> c.spt >> c.chunk_particles >> c.listed_grid) g_tune_cache[key] = c;
+}
+
+void save_tune_entry(const std::string& key, const LaunchConfig& c) {
+    const char* path = std::getenv("LIMA_AUTOTUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << "\n";
 }
 
 void get3(const std::vector<float4>& v, float* out) {
@@ -948,18 +1026,45 @@ int lima_set_neighbors(lima_engine* E, const int32_t* ccc, const int32_t* cci, c
     return guarded(E, [&] {
         HIP_TRY(hipStreamSynchronize(E->stream));
         const lima_run_params& P = E->params;
+        // every id must name an entity of the same replica (the kernels index per-replica ranges)
+        std::vector<int> comp_rep_h(E->NC), solv_rep_h(E->NS);
+        for (int r = 0; r < E->R; r++) {
+            for (int c = E->reps_h[r].comp_begin; c < E->reps_h[r].comp_end; c++) comp_rep_h[c] = r;
+            for (int s = E->reps_h[r].solv_begin; s < E->reps_h[r].solv_end; s++) solv_rep_h[s] = r;
+        }
         if (ccc && cci && E->NC) {
-            for (int c = 0; c < E->NC; c++) if (ccc[c] < 0 || ccc[c] > P.max_compound_neighbors) throw LimaError(LIMA_E_CAPACITY, "compound list count exceeds capacity");
+            for (int c = 0; c < E->NC; c++) {
+                if (ccc[c] < 0 || ccc[c] > P.max_compound_neighbors) throw LimaError(LIMA_E_CAPACITY, "compound list count exceeds capacity");
+                const ReplicaInfo& ri = E->reps_h[comp_rep_h[c]];
+                for (int k = 0; k < ccc[c]; k++) {
+                    const int j = cci[(size_t)c * P.max_compound_neighbors + k];
+                    if (j < ri.comp_begin || j >= ri.comp_end || j == c) throw LimaError(LIMA_E_ARG, "compound neighbour id outside the compound's system");
+                }
+            }
             HIP_TRY(hipMemcpy(E->cc_cnt.p, ccc, E->NC * sizeof(int), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(E->cc_ids.p, cci, (size_t)E->NC * P.max_compound_neighbors * sizeof(int), hipMemcpyHostToDevice));
         }
         if (csc && csi && E->NC) {
-            for (int c = 0; c < E->NC; c++) if (csc[c] < 0 || csc[c] > P.max_compound_solvents) throw LimaError(LIMA_E_CAPACITY, "compound solvent list count exceeds capacity");
+            for (int c = 0; c < E->NC; c++) {
+                if (csc[c] < 0 || csc[c] > P.max_compound_solvents) throw LimaError(LIMA_E_CAPACITY, "compound solvent list count exceeds capacity");
+                const ReplicaInfo& ri = E->reps_h[comp_rep_h[c]];
+                for (int k = 0; k < csc[c]; k++) {
+                    const int s = csi[(size_t)c * P.max_compound_solvents + k];
+                    if (s < ri.solv_begin || s >= ri.solv_end) throw LimaError(LIMA_E_ARG, "solvent id outside the compound's system");
+                }
+            }
             HIP_TRY(hipMemcpy(E->cs_cnt.p, csc, E->NC * sizeof(int), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(E->cs_ids.p, csi, (size_t)E->NC * P.max_compound_solvents * sizeof(int), hipMemcpyHostToDevice));
         }
         if (ssc && ssi && E->NS) {
-            for (int s = 0; s < E->NS; s++) if (ssc[s] < 0 || ssc[s] > P.max_solvent_neighbors) throw LimaError(LIMA_E_CAPACITY, "solvent list count exceeds capacity");
+            for (int s = 0; s < E->NS; s++) {
+                if (ssc[s] < 0 || ssc[s] > P.max_solvent_neighbors) throw LimaError(LIMA_E_CAPACITY, "solvent list count exceeds capacity");
+                const ReplicaInfo& ri = E->reps_h[solv_rep_h[s]];
+                for (int k = 0; k < ssc[s]; k++) {
+                    const int j = ssi[(size_t)s * P.max_solvent_neighbors + k];
+                    if (j < ri.solv_begin || j >= ri.solv_end || j == s) throw LimaError(LIMA_E_ARG, "solvent neighbour id outside the solvent's system");
+                }
+            }
             std::vector<int> t((size_t)E->NS * P.max_solvent_neighbors, 0);
             for (int s = 0; s < E->NS; s++)
                 for (int k = 0; k < ssc[s]; k++) t[(size_t)k * E->NS + s] = ssi[(size_t)s * P.max_solvent_neighbors + k];
@@ -1058,9 +1163,79 @@ int lima_get_config(lima_engine* E, int32_t* out8) {
     });
 }
 
-int lima_autotune(lima_engine* E, int32_t trial_steps) {
-    (void)trial_steps;
-    return guarded(E, [&] { return (int)LIMA_E_UNSUPPORTED; });
+// Autotuner: picks the all-pairs kernel's solvents per thread and chunk size, and
+// the listed kernel's grid, by timing each candidate alone on the current state
+// (state and lists are not modified).  Results are cached per shape histogram in
+// the process and, when LIMA_AUTOTUNE_CACHE names a file, across processes.
+int lima_autotune(lima_engine* E, int32_t reps) {
+    return guarded(E, [&] {
+        if (reps <= 0) reps = 10;
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        const std::string key = shape_key(E);
+        LaunchConfig best = E->cfg;
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            load_tune_file();
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) { best = it->second; have = true; }
+        }
+        if (!have) {
+            hipEvent_t e0, e1;
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+            auto timeit = [&](auto&& launch) -> double {
+                launch();
+                HIP_TRY(hipEventRecord(e0, E->stream));
+                for (int i = 0; i < reps; i++) launch();
+                HIP_TRY(hipEventRecord(e1, E->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                return (double)ms / reps;
+            };
+            const bool fx = E->params.accumulation == LIMA_ACCUM_FIXED64;
+            double tbest = 1e30;
+            if (E->params.enable_solvents && E->NS > 0 && E->NCH > 0) {
+                for (int spt : {1, 2, 4}) {
+                    for (int chunk : {256, 512, 1024, 2048}) {
+                        E->cfg.spt = spt;
+                        E->cfg.chunk_particles = chunk;
+                        E->build_scwork();
+                        E->reps.upload(E->reps_h);
+                        StepArgs a = E->step_args(E->cur);
+                        const double t = timeit([&] {
+                            if (fx) E->launch_solvent_chain<true, false>(a, E->stream);
+                            else E->launch_solvent_chain<false, false>(a, E->stream);
+                        });
+                        if (t < tbest) { tbest = t; best.spt = spt; best.chunk_particles = chunk; }
+                    }
+                }
+            }
+            if (E->NC > 0) {
+                double gbest = 1e30;
+                StepArgs a = E->step_args(E->cur);
+                for (int grid : {1024, 2048, 4096, 8192}) {
+                    const double t = timeit([&] {
+                        if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                        else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                    });
+                    if (t < gbest) { gbest = t; best.listed_grid = grid; }
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune_cache[key] = best;
+            save_tune_entry(key, best);
+        }
+        E->cfg = best;
+        E->build_scwork();
+        E->reps.upload(E->reps_h);
+        if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+        return E->check_errors();
+    });
 }
 
 }  // extern "C"

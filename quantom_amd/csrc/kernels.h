@@ -60,11 +60,12 @@ struct StepArgs {
     const int4* stiles;           // (replica, begin, end, tile index) per solvent-integrate tile
     // type tables per replica
     const float* mass;            // [R*32]
-    const float2* tab_sc;         // [R*32] solvent <- chain type (sig2, 24 eps), arithmetic eps
-    const float2* tab_cs;         // [R*32] chain type <- solvent, geometric eps
-    const float2* tab_inter;      // [R*1024] arithmetic
-    const float2* tab_intra;      // [R*1024] geometric
-    const float2* tab_ss;         // [R]
+    // pair parameters (sigma^2, 24 eps, -48 eps, 0) with each call site's mixing rule
+    const float4* tab_sc;         // [R*32] solvent <- chain type, arithmetic eps
+    const float4* tab_cs;         // [R*32] chain type <- solvent, geometric eps
+    const float4* tab_inter;      // [R*1024] arithmetic
+    const float4* tab_intra;      // [R*1024] geometric
+    const float4* tab_ss;         // [R]
     // lists
     int cap_cc, cap_cs, cap_ss;
     const int* cc_cnt; const int* cc_ids;
@@ -77,6 +78,7 @@ struct StepArgs {
     float4* partC;  I3* partC_fx;   // [slot*128 + p]
     float4* partS;  I3* partS_fx;   // [chunk*n_solv + s]
     const SCWork* scwork; int n_scwork;
+    int chunk_cap;                // chain particles of the largest solvent<-chain chunk (LDS tile size)
     // thermostat
     const float* lambda;          // [R] applied this step
     double* ke_comp;              // [n_comp]
@@ -177,10 +179,12 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 // ---------------------------------------------------------------------------
 template <int SPT, bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
-    __shared__ float4 tile[kChunkParticles];
-    __shared__ float tile_e[kChunkParticles];
+    // dynamic LDS sized to the largest chunk of this engine: [cap] float4 + [cap] float2
+    extern __shared__ float4 dyn_lds[];
+    float4* tile = dyn_lds;
+    float2* tile_e = reinterpret_cast<float2*>(dyn_lds + a.chunk_cap);
     __shared__ int tile_off[kChunkCompounds + 1];
-    __shared__ float2 tab[kMaxTypes];
+    __shared__ float4 tab[kMaxTypes];
     const SCWork wk = a.scwork[blockIdx.x];
     const ReplicaInfo rp = a.reps[wk.replica];
     const float L = rp.L, half = rp.half;
@@ -192,9 +196,9 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     __syncthreads();
     for (int k = tid; k < total; k += blockDim.x) {
         float4 p = a.cpos[base + k];
-        float2 tp = tab[ftype(p)];
+        float4 tp = tab[ftype(p)];
         tile[k] = make_float4(p.x, p.y, p.z, tp.x);
-        tile_e[k] = tp.y;
+        tile_e[k] = make_float2(tp.y, tp.z);
     }
     float sx[SPT], sy[SPT], sz[SPT];
     float fx[SPT], fy[SPT], fz[SPT];
@@ -227,11 +231,11 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 #pragma unroll 4
         for (int j = j0; j < j1; j++) {
             const float4 q = tile[j];
-            const float e = tile_e[j];
+            const float2 e = tile_e[j];
 #pragma unroll
             for (int k = 0; k < SPT; k++) {
-                if (ENERGY) lj_acc_energy(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e, lx[k], ly[k], lz[k], en[k], vir[k]);
-                else lj_acc(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e, lx[k], ly[k], lz[k]);
+                if (ENERGY) lj_acc_energy(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e.x, e.y, lx[k], ly[k], lz[k], en[k], vir[k]);
+                else lj_acc(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e.x, e.y, lx[k], ly[k], lz[k]);
             }
         }
 #pragma unroll
@@ -277,7 +281,7 @@ LIMA_DEV int find_compound(const int* cw_off, int n, int w) {
 template <bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
     __shared__ float4 src[kSolventChunk];
-    __shared__ float2 tab[kMaxTypes * kMaxTypes];
+    __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (sigma^2, 24 eps): 8-byte entries keep the per-lane reads conflict-light
     __shared__ int cached_rep;
     const int p = threadIdx.x;
     if (p == 0) cached_rep = -1;
@@ -300,7 +304,10 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
         double en = 0.0, vir = 0.0;
         if (k < ncc) {
             if (cached_rep != r) {
-                for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) tab[t] = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + t];
+                for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
+                    const float4 v = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + t];
+                    tab[t] = make_float2(v.x, v.y);
+                }
             }
             const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
             const int nbase = a.comp_off[nb], nn = a.comp_off[nb + 1] - nbase;
@@ -333,10 +340,8 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                         const float2 pr = tab[ftype(q) * kMaxTypes + tp];
                         float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
                         float r2 = dx * dx + dy * dy + dz * dz;
-                        float inv = __builtin_amdgcn_rcpf(r2);
-                        float s = pr.x * inv;
-                        float s3 = s * s * s;
-                        float fs = pr.y * s3 * inv * (1.0f - 2.0f * s3);
+                        float inv;
+                        float fs = lj_scalar(r2, pr.x, pr.y, -2.0f * pr.y, inv);
                         const bool ex = (mw >> (j & 31)) & 1u;
                         fs = ex ? 0.0f : fs;
                         lx += dx * fs; ly += dy * fs; lz += dz * fs;
@@ -362,12 +367,12 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                 src[jj] = q;
             }
             __syncthreads();
-            const float2 pr = a.tab_cs[r * kMaxTypes + tp];
+            const float4 pr = a.tab_cs[r * kMaxTypes + tp];
             if (active) {
                 for (int j = 0; j < nsrc; j++) {
                     const float4 q = src[j];
-                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz, en, vir);
-                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz);
+                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, pr.z, lx, ly, lz, en, vir);
+                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, pr.z, lx, ly, lz);
                 }
             }
             if (ENERGY) {
@@ -410,7 +415,10 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
     const float4 p0 = a.cpos[base];
     hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);                 // Engine.cu:923
     pos[p] = me;
-    for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) tab[t] = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + t];
+    for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
+        const float4 v = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + t];
+        tab[t] = make_float2(v.x, v.y);
+    }
     __syncthreads();
     double eb = 0, ea = 0, ed = 0;
     const int b0 = a.bond_off[c], nb = a.bond_off[c + 1] - b0;
@@ -469,10 +477,8 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
                 const float2 pr = tab[ftype(q) * kMaxTypes + tp];
                 float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
-                float inv = __builtin_amdgcn_rcpf(r2);
-                float s = pr.x * inv;
-                float s3 = s * s * s;
-                float fs = pr.y * s3 * inv * (1.0f - 2.0f * s3);
+                float inv;
+                float fs = lj_scalar(r2, pr.x, pr.y, -2.0f * pr.y, inv);
                 const bool ex = (j == p) || ((mw >> (j & 31)) & 1u);
                 fs = ex ? 0.0f : fs;
                 lx += dx * fs; ly += dy * fs; lz += dz * fs;
@@ -564,7 +570,7 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
         float4 q = a.cpos[a.comp_off[c]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
     }
-    const float2 pss = a.tab_ss[r];
+    const float4 pss = a.tab_ss[r];
     float lx = 0.f, ly = 0.f, lz = 0.f;
     double en = 0, vir = 0;
     const int cnt = active ? a.ss_cnt[si] : 0;
@@ -577,15 +583,15 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
-            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
-            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
+            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, pss.z, lx, ly, lz, en, vir);
+            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, pss.z, lx, ly, lz);
         }
     }
     for (; k < cnt; k++) {
         const float4 q = a.spos[ids[k * ld]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
-        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
+        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, pss.z, lx, ly, lz, en, vir);
+        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, pss.z, lx, ly, lz);
     }
     if (active) {
         a.ss_part[s] = make_float4(lx, ly, lz, 0.f);
@@ -714,13 +720,15 @@ struct NlistArgs {
     const int* solv_rep;
     float cutoff;
     int cap_cc, cap_cs, cap_ss;
-    int* cc_cnt; int* cc_ids; int* cc_old_cnt; int* cc_old_ids;
+    int* cc_cnt; int* cc_ids;
     int* cs_cnt; int* cs_ids;
     int* ss_cnt; int* ss_ids;
     int n_cells;
     int* cell_cnt; int* cell_start; int* cell_cursor; int* cell_ids; int* solv_cell;
     const int* cell_rep;          // replica of each cell
     const float* comp_cs_bound;   // per compound: bound(C + (r + C)) for the squared distance
+    int* seg_cnt;                 // [n_comp * max_segs] compound-solvent members per solvent segment
+    int max_segs;
     int* cw_off; int* cw_total;
     uint32_t* err;
     int enable_solvents;
@@ -728,24 +736,32 @@ struct NlistArgs {
 
 LIMA_DEV float4 key_pos(const NlistArgs& a, int c) { return a.cpos[a.comp_off[c] + a.comp_key[c]]; }
 
-__global__ void nl_copy_cc_kernel(NlistArgs a) {
-    const int c = blockIdx.x;
-    const int cnt = a.cc_cnt[c];
-    if (threadIdx.x == 0) a.cc_old_cnt[c] = cnt;
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) a.cc_old_ids[(size_t)c * a.cap_cc + k] = a.cc_ids[(size_t)c * a.cap_cc + k];
-}
+// One wavefront per compound, candidates of its replica in ascending id order.
+// The compound's previous list is first turned into an LDS bitmask (it is the
+// only reader of its own list), then overwritten with the new one.
+constexpr int kMaxReplicaCompounds = 8192;
 
-// one wavefront per compound, candidates in ascending id order
 __global__ __launch_bounds__(64) void nl_compound_compound_kernel(NlistArgs a) {
+    __shared__ uint32_t was_bits[kMaxReplicaCompounds / 32];
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int r = a.comp_rep[i];
     const ReplicaInfo rp = a.reps[r];
+    const int ncr = rp.comp_end - rp.comp_begin;
+    if (ncr > kMaxReplicaCompounds) {
+        if (lane == 0) atomicOr(a.err, kErrCapacity);
+        return;
+    }
     const float L = rp.L, half = rp.half, C = a.cutoff;
+    for (int w = lane; w < (ncr + 31) / 32; w += 64) was_bits[w] = 0u;
+    __syncthreads();
+    const int old_cnt = a.cc_cnt[i];
+    for (int e = lane; e < old_cnt; e += 64) {
+        const int j = a.cc_ids[(size_t)i * a.cap_cc + e] - rp.comp_begin;
+        atomicOr(&was_bits[j >> 5], 1u << (j & 31));
+    }
+    __syncthreads();
     const float4 ki = key_pos(a, i);
-    const float ri = a.comp_sphere[i];
-    const int old_cnt = a.cc_old_cnt[i];
-    const int* old = a.cc_old_ids + (size_t)i * a.cap_cc;
     int count = 0;
     for (int j0 = rp.comp_begin; j0 < rp.comp_end; j0 += 64) {
         const int j = j0 + lane;
@@ -755,8 +771,8 @@ __global__ __launch_bounds__(64) void nl_compound_compound_kernel(NlistArgs a) {
             const float4 klo = lo == i ? ki : key_pos(a, j);
             const float4 khi = hi == i ? ki : key_pos(a, j);
             const float rlo = a.comp_sphere[lo], rhi = a.comp_sphere[hi];
-            bool was = false;
-            for (int e = 0; e < old_cnt; e++) was |= (old[e] == j);
+            const int jl = j - rp.comp_begin;
+            const bool was = (was_bits[jl >> 5] >> (jl & 31)) & 1u;
             const bool keep = within(klo.x, klo.y, klo.z, khi.x, khi.y, khi.z, fadd(fadd(rlo, rhi), C), C, L, half);
             const bool add = within(klo.x, klo.y, klo.z, khi.x, khi.y, khi.z, fadd(fadd(rlo, rlo), C), C, L, half);
             member = (was && keep) || add;
@@ -769,48 +785,70 @@ __global__ __launch_bounds__(64) void nl_compound_compound_kernel(NlistArgs a) {
         }
         count += __popcll(ball);
     }
-    (void)ri;
     if (lane == 0) {
         if (count > a.cap_cc) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 1, count); }
         a.cc_cnt[i] = min(count, a.cap_cc);
     }
 }
 
-// one block (256) per compound, solvents of the replica in ascending order
-__global__ __launch_bounds__(256) void nl_compound_solvent_kernel(NlistArgs a) {
-    __shared__ int wave_cnt[4];
-    const int i = blockIdx.x;
-    const int r = a.comp_rep[i];
-    const ReplicaInfo rp = a.reps[r];
-    const float L = rp.L, half = rp.half, C = a.cutoff;
-    const float4 ki = key_pos(a, i);
+// Compound -> solvent lists in ascending solvent order, in two passes over a
+// (compound, 2048-solvent segment) grid: count members per segment, then write
+// each segment's members at the prefix of the counts before it.
+constexpr int kSegSolvents = 2048;
+
+LIMA_DEV bool cs_member(const NlistArgs& a, float4 ki, float bound, int s, float L, float half) {
+    const float4 q = a.spos[s];
     // |d| < C + (r + C)  <=>  d^2 < bound (correctly rounded sqrt is monotone)
+    return dist2_reimaged(ki.x, ki.y, ki.z, q.x, q.y, q.z, L, half) < bound;
+}
+
+__global__ __launch_bounds__(256) void nl_cs_count_kernel(NlistArgs a) {
+    __shared__ int wsum[4];
+    const int i = blockIdx.x, seg = blockIdx.y;
+    const ReplicaInfo rp = a.reps[a.comp_rep[i]];
+    const int s0 = rp.solv_begin + seg * kSegSolvents;
+    const int s1 = min(s0 + kSegSolvents, rp.solv_end);
+    const float4 ki = key_pos(a, i);
     const float bound = a.comp_cs_bound[i];
-    (void)C;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int c = 0;
+    for (int s = s0 + (int)threadIdx.x; s < s1; s += blockDim.x) c += cs_member(a, ki, bound, s, rp.L, rp.half);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) a.seg_cnt[(size_t)i * a.max_segs + seg] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void nl_cs_write_kernel(NlistArgs a) {
+    __shared__ int wave_cnt[4];
+    const int i = blockIdx.x, seg = blockIdx.y;
+    const ReplicaInfo rp = a.reps[a.comp_rep[i]];
+    const int nseg = (rp.solv_end - rp.solv_begin + kSegSolvents - 1) / kSegSolvents;
+    const int* sc = a.seg_cnt + (size_t)i * a.max_segs;
+    if (seg == 0 && threadIdx.x == 0) {
+        int total = 0;
+        for (int k = 0; k < nseg; k++) total += sc[k];
+        if (total > a.cap_cs) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 2, total); }
+        a.cs_cnt[i] = min(total, a.cap_cs);
+    }
+    if (seg >= nseg) return;
     int count = 0;
-    for (int s0 = rp.solv_begin; s0 < rp.solv_end; s0 += 256) {
-        const int s = s0 + threadIdx.x;
-        bool member = false;
-        if (s < rp.solv_end) {
-            const float4 q = a.spos[s];
-            member = dist2_reimaged(ki.x, ki.y, ki.z, q.x, q.y, q.z, L, half) < bound;
-        }
+    for (int k = 0; k < seg; k++) count += sc[k];
+    const int s0 = rp.solv_begin + seg * kSegSolvents;
+    const int s1 = min(s0 + kSegSolvents, rp.solv_end);
+    const float4 ki = key_pos(a, i);
+    const float bound = a.comp_cs_bound[i];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int b = s0; b < s1; b += 256) {
+        const int s = b + threadIdx.x;
+        const bool member = s < s1 && cs_member(a, ki, bound, s, rp.L, rp.half);
         const unsigned long long ball = __ballot(member);
         if (lane == 0) wave_cnt[wid] = __popcll(ball);
         __syncthreads();
         int before = __popcll(ball & ((1ull << lane) - 1ull));
         for (int w = 0; w < wid; w++) before += wave_cnt[w];
-        if (member) {
-            const int pos = count + before;
-            if (pos < a.cap_cs) a.cs_ids[(size_t)i * a.cap_cs + pos] = s;
-        }
+        if (member && count + before < a.cap_cs) a.cs_ids[(size_t)i * a.cap_cs + count + before] = s;
         count += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (count > a.cap_cs) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 2, count); }
-        a.cs_cnt[i] = min(count, a.cap_cs);
     }
 }
 
@@ -903,7 +941,7 @@ LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, 
     return (j != s) && dist2_reimaged(ax, ay, az, bx, by, bz, L, half) < bound;
 }
 
-__global__ __launch_bounds__(128) void nl_solvent_solvent_kernel(NlistArgs a) {
+__global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
     __shared__ float4 cand_pos[kCellCandidates];
     __shared__ int cand_id[kCellCandidates];
     const int cell = blockIdx.x;
@@ -940,19 +978,35 @@ __global__ __launch_bounds__(128) void nl_solvent_solvent_kernel(NlistArgs a) {
         total += e - b;
     }
     __syncthreads();
+    if (total <= kCellCandidates) {
+        // each wavefront takes home solvents in turn; its lanes test 64 candidates
+        // at a time and append the members in candidate order (ballot compaction)
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int h = h0 + wid; h < h1; h += nw) {
+            const int s = a.cell_ids[h];
+            const float4 me = a.spos[s];
+            int count = 0;
+            for (int e0 = 0; e0 < total; e0 += 64) {
+                const int e = e0 + lane;
+                const int j = e < total ? cand_id[e] : s;
+                const bool member = e < total && ss_member(s, me, j, cand_pos[e < total ? e : 0], L, half, bound);
+                const unsigned long long ball = __ballot(member);
+                const int pos = count + __popcll(ball & ((1ull << lane) - 1ull));
+                if (member && pos < a.cap_ss) a.ss_ids[(size_t)pos * a.n_solv + s] = j;
+                count += __popcll(ball);
+            }
+            if (lane == 0) {
+                if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
+                a.ss_cnt[s] = min(count, a.cap_ss);
+            }
+        }
+        return;
+    }
     for (int h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
         const int s = a.cell_ids[h];
         const float4 me = a.spos[s];
         int count = 0;
-        if (total <= kCellCandidates) {
-            for (int e = 0; e < total; e++) {
-                const int j = cand_id[e];
-                if (ss_member(s, me, j, cand_pos[e], L, half, bound)) {
-                    if (count < a.cap_ss) a.ss_ids[(size_t)count * a.n_solv + s] = j;
-                    count++;
-                }
-            }
-        } else {   // more candidates than LDS holds: same order, read from global memory
+        {   // more candidates than LDS holds: same order, read from global memory
             for (int k = 0; k < nnb; k++) {
                 const int e1 = a.cell_start[nb[k] + 1];
                 for (int e = a.cell_start[nb[k]]; e < e1; e++) {
