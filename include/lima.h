@@ -212,6 +212,49 @@ int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
 int lima_get_config(lima_engine* engine, int32_t* out8);
 int lima_autotune(lima_engine* engine, int32_t trial_steps);
 
+/* ---- parameters and reverse-mode gradients of trajectory losses.
+ * The reference has no differentiable path; these calls take the same force
+ * field (ForceField, Forcefield.cuh:12-27) and bonded constants (PairBond /
+ * AngleBond, Bodies.cuh:61-88) as lima_create and expose them as a parameter
+ * vector per system:
+ *   [0..31]  sigma of types 0..31      [32..63] epsilon of types 0..31
+ *   [64] kb  [65] b0  [66] k_theta  [67] theta_0
+ * The four bonded entries act as constants shared by every pair bond / angle of
+ * the system (bridges included): setting them writes every term, and their
+ * gradient is the sum of the per-term derivatives. */
+#define LIMA_N_PARAMETERS 68
+
+int lima_get_parameters(lima_engine* engine, int32_t system, double* params);
+int lima_set_parameters(lima_engine* engine, int32_t system, const double* params);
+
+/* Loss over the frames x_0 .. x_{K-1} of the trajectory that starts at the
+ * current state (x_0 = current positions, integrated with the engine's step;
+ * the thermostat must be LIMA_THERMOSTAT_NONE):
+ *   L_r = sum_k alpha_r (E_k,r - E*_k,r)^2 + beta_r sum_i |F_k,i - F*_k,i|^2
+ * with E the potential energy of lima_energies and F the force of each particle
+ * on frame k.  NULL weights or targets read as zeros. */
+typedef struct lima_adjoint_spec {
+    int32_t n_frames;                  /* K >= 1 */
+    int32_t checkpoint_interval;       /* frames per stored state; 0 = 20.  Segments start on list rebuilds. */
+    const double* energy_weight;       /* [n_systems] alpha */
+    const double* force_weight;        /* [n_systems] beta */
+    const double* energy_target;       /* [K][n_systems] E* (J/mol) */
+    const float* chain_force_target;   /* [K][n_chain][3] F*, systems concatenated */
+    const float* solvent_force_target; /* [K][n_solvents][3] */
+} lima_adjoint_spec;
+
+/* Forward pass over the K frames, then the adjoint recurrence backwards with
+ * checkpoints and recomputation of each segment (neighbour lists restored
+ * exactly).  loss: [n_systems]; grad: [n_systems][LIMA_N_PARAMETERS] or NULL
+ * (loss only).  The engine's state, lists and step count are left as they were. */
+int lima_adjoint_run(lima_engine* engine, const lima_adjoint_spec* spec, double* loss, double* grad);
+
+/* The same K frames, recorded: forces [K][n_chain][3], [K][n_solvents][3] and
+ * potential energies [K][n_systems] (any output may be NULL).  The engine's
+ * state is left as it was. */
+int lima_record_frames(lima_engine* engine, int32_t n_frames, float* chain_forces, float* solvent_forces,
+                       double* potential);
+
 #ifdef __cplusplus
 }
 #endif

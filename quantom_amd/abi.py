@@ -99,6 +99,17 @@ class CEnergies(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class CAdjointSpec(C.Structure):
+    """lima_adjoint_spec: frames, checkpoint interval, weights and targets (NULL = zeros)."""
+    _fields_ = [("n_frames", C.c_int32), ("checkpoint_interval", C.c_int32),
+                ("energy_weight", C.POINTER(C.c_double)), ("force_weight", C.POINTER(C.c_double)),
+                ("energy_target", C.POINTER(C.c_double)), ("chain_force_target", C.POINTER(C.c_float)),
+                ("solvent_force_target", C.POINTER(C.c_float))]
+
+
+N_PARAMETERS = 68   # sigma[32], epsilon[32], kb, b0, k_theta, theta_0
+
+
 @dataclass
 class RunParams:
     """Runtime restatement of the Constants.cuh switches and capacities."""

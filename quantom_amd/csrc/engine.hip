@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/lima.h"
+#include "adjoint.h"
 #include "kernels.h"
 
 using namespace lima;
@@ -45,6 +46,10 @@ struct LimaError {
 template <typename T> struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { free(); }
     void alloc(size_t count) {
         free();
         n = count;
@@ -71,6 +76,25 @@ struct LaunchConfig {
     int use_graph = 1;
 };
 
+// Device buffers of lima_adjoint_run / lima_record_frames, grown on demand and
+// kept with the engine.  N = chain + solvent particles, chain first.
+struct AdjBufs {
+    DBuf<float4> ck;          // [segments][2N]: chain pos, chain prev, solvent pos, solvent prev
+    DBuf<int> ck_cc;          // [segments][NC + NC*cap_cc]: compound lists before the segment's rebuild
+    DBuf<int> lists;          // [slots][list words]: slot 0 the entry lists, then a segment's periods
+    DBuf<float4> X, Rs, T;    // [segment frames][N]: positions, 2 beta (F - F*), targets F*
+    DBuf<float4> adj;         // [3][N] a_k, a_{k+1}, a_{k+2} in rotation
+    DBuf<float4> w;           // [N]
+    DBuf<float4> gxc, gxs;    // position-gradient parts
+    DBuf<float> cfac, gE;     // [N] dt^2/m, [K][R] energy weights
+    DBuf<int> prep;           // [N] system of each particle
+    DBuf<double> beta, loss, gpar, efr;
+    DBuf<uint32_t> mask;
+    DBuf<int> mask_off;
+    int n_cchunks = 1, n_sslices = 1;
+    bool ready = false;
+};
+
 }  // namespace
 
 struct lima_engine {
@@ -88,6 +112,11 @@ struct lima_engine {
     std::vector<SCWork> scwork_h;
     std::vector<int> comp_off_h;
     int device = 0;
+    // host copies behind lima_get/set_parameters
+    std::vector<lima_forcefield> ff_h;
+    std::vector<BondD> bonds_h, brb_h;
+    std::vector<AngleD> angles_h, bra_h;
+    std::vector<int> bond_off_h, angle_off_h, brb_off_h, bra_off_h, br_rep_h;
 
     // ---- device buffers ----
     DBuf<ReplicaInfo> reps;
@@ -123,6 +152,7 @@ struct lima_engine {
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
     DBuf<float4> ss_part, ss_img;
     DBuf<float> comp_cs_bound;
+    AdjBufs adjb;
 
     // the step is a fork/join DAG over three streams (all at default priority):
     // stream: solvent<-chain all-pairs -> solvent integrate -> thermostat [-> rebuild]
@@ -329,9 +359,17 @@ struct lima_engine {
         HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
     }
 
-    void launch_step(int parity) {
-        if (params.accumulation == LIMA_ACCUM_FIXED64) launch_forces<true, true, false>(parity);
-        else launch_forces<true, false, false>(parity);
+    // one step; with `energy` the force pass also accumulates the FP64 energy
+    // terms (the caller clears the energy buffer first)
+    void launch_step(int parity, bool energy = false) {
+        const bool fx = params.accumulation == LIMA_ACCUM_FIXED64;
+        if (energy) {
+            if (fx) launch_forces<true, true, true>(parity);
+            else launch_forces<true, false, true>(parity);
+        } else {
+            if (fx) launch_forces<true, true, false>(parity);
+            else launch_forces<true, false, false>(parity);
+        }
         // with solvents disabled the solvent state is carried across the ping-pong unchanged
         if (!params.enable_solvents && NS > 0)
             HIP_TRY(hipMemcpyAsync(spos[parity ^ 1].p, spos[parity].p, NS * sizeof(float4), hipMemcpyDeviceToDevice, stream));
@@ -459,10 +497,34 @@ float sqrt_bound(float t) {
 }
 
 // LJ parameters of one call site: sigma^2 and 24*eps rounded as the reference
-// rounds sigma*sigma and 24.f*epsilon (Engine.cu:488-490), and -2*(24 eps).
+// rounds sigma*sigma and 24.f*epsilon (Engine.cu:488-490), -2*(24 eps), and the
+// mixed sigma itself (d sigma^2 / d sigma_type = sigma for the arithmetic sigma rule).
 float4 pair_params(float sigma, float eps) {
     const float e24 = 24.f * eps;
-    return make_float4(sigma * sigma, e24, -2.0f * e24, 0.0f);
+    return make_float4(sigma * sigma, e24, -2.0f * e24, sigma);
+}
+
+// type tables of one system (mixing per call site, Engine.cu:643-646, 668-670,
+// 689-691, 711-713, 724-727): [32] masses, [32] solvent <- type, [32] type <-
+// solvent, [32*32] inter-compound, [32*32] intra-compound, [1] solvent-solvent
+void fill_tables(const lima_particle_params* pp, float* mass, float4* tsc, float4* tcs, float4* tinter, float4* tintra,
+                 float4* tss) {
+    for (int t = 0; t < kMaxTypes; t++) {
+        mass[t] = pp[t].mass;
+        float sg = (pp[0].sigma + pp[t].sigma) * 0.5f;
+        float ep = (pp[0].epsilon + pp[t].epsilon) * 0.5f;
+        tsc[t] = pair_params(sg, ep);
+        float epg = sqrtf(pp[t].epsilon * pp[0].epsilon);
+        tcs[t] = pair_params(sg, epg);
+        for (int u = 0; u < kMaxTypes; u++) {
+            float su = (pp[t].sigma + pp[u].sigma) * 0.5f;
+            float ea = (pp[t].epsilon + pp[u].epsilon) * 0.5f;
+            float eg = sqrtf(pp[t].epsilon * pp[u].epsilon);
+            tinter[t * 32 + u] = pair_params(su, ea);
+            tintra[t * 32 + u] = pair_params(su, eg);
+        }
+    }
+    *tss = pair_params(pp[0].sigma, pp[0].epsilon);
 }
 
 void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
@@ -492,6 +554,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     std::vector<float> mass(nsys * kMaxTypes);
     std::vector<float4> tsc(nsys * kMaxTypes), tcs(nsys * kMaxTypes), tinter(nsys * 1024), tintra(nsys * 1024), tss(nsys);
     E->reps_h.clear();
+    E->ff_h.clear();
     E->sys_chain_begin.clear(); E->sys_solv_begin.clear(); E->sys_comp_begin.clear();
     std::vector<int> bridge_owner;
 
@@ -515,24 +578,9 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
         const int nchain = s.n_compounds > 0 ? s.compound_offset[s.n_compounds] : 0;
         ri.chain_begin = chb; ri.chain_end = chb + nchain;
         ri.solv_begin = sb; ri.solv_end = sb + s.n_solvents;
-        // type tables (mixing per call site, Engine.cu:643-646, 668-670, 689-691, 711-713, 724-727)
-        const lima_particle_params* pp = s.forcefield.particle_parameters;
-        for (int t = 0; t < kMaxTypes; t++) {
-            mass[r * kMaxTypes + t] = pp[t].mass;
-            float sg = (pp[0].sigma + pp[t].sigma) * 0.5f;
-            float ep = (pp[0].epsilon + pp[t].epsilon) * 0.5f;
-            tsc[r * kMaxTypes + t] = pair_params(sg, ep);
-            float epg = sqrtf(pp[t].epsilon * pp[0].epsilon);
-            tcs[r * kMaxTypes + t] = pair_params(sg, epg);
-            for (int u = 0; u < kMaxTypes; u++) {
-                float su = (pp[t].sigma + pp[u].sigma) * 0.5f;
-                float ea = (pp[t].epsilon + pp[u].epsilon) * 0.5f;
-                float eg = sqrtf(pp[t].epsilon * pp[u].epsilon);
-                tinter[r * 1024 + t * 32 + u] = pair_params(su, ea);
-                tintra[r * 1024 + t * 32 + u] = pair_params(su, eg);
-            }
-        }
-        tss[r] = pair_params(pp[0].sigma, pp[0].epsilon);
+        fill_tables(s.forcefield.particle_parameters, &mass[r * kMaxTypes], &tsc[r * kMaxTypes], &tcs[r * kMaxTypes],
+                    &tinter[r * 1024], &tintra[r * 1024], &tss[r]);
+        E->ff_h.push_back(s.forcefield);
         // compounds
         for (int c = 0; c < s.n_compounds; c++) {
             const int o0 = s.compound_offset[c], o1 = s.compound_offset[c + 1];
@@ -702,6 +750,9 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->brb_off.upload(brb_off); E->brb.upload(brb);
     E->bra_off.upload(bra_off); E->bra.upload(bra);
     E->brd_off.upload(brd_off); E->brd.upload(brd);
+    E->bonds_h = bonds; E->angles_h = angles; E->brb_h = brb; E->bra_h = bra;
+    E->bond_off_h = bond_off; E->angle_off_h = angle_off; E->brb_off_h = brb_off; E->bra_off_h = bra_off;
+    E->br_rep_h = br_rep;
     for (int k = 0; k < 2; k++) E->spos[k].upload(spos);
     E->sprev.upload(sprev);
     E->sforce.alloc(std::max(E->NS, 1)); E->sforce.zero();
@@ -744,6 +795,528 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->comp_cs_bound.upload(cs_bound);
     E->ss_part.alloc(NSx); E->ss_part.zero();
     E->ss_img.alloc(NSx); E->ss_img.zero();
+}
+
+// ---- reverse mode ------------------------------------------------------------
+// Frames k = 0..K-1 start at the current state.  A forward pass stores a state
+// checkpoint at the start of every segment (segments begin on list rebuilds,
+// every `checkpoint_interval` frames or more) together with the compound lists
+// the segment's first rebuild reads (its hysteresis input), and evaluates the
+// loss.  The backward pass visits the segments last to first: it restores the
+// checkpoint, rebuilds the segment's first lists from the stored compound lists
+// (the same kernels on the same input give the same lists), recomputes the
+// segment's frames keeping positions and force residuals (and a full copy of
+// the lists before each rebuild inside the segment), then runs the adjoint
+// recurrence over the segment's frames in reverse, restoring each period's lists.
+
+size_t list_words(const lima_engine* E) {
+    const lima_run_params& P = E->params;
+    return (size_t)E->NC * (2 + P.max_compound_neighbors + P.max_compound_solvents) +
+           (size_t)E->NS * (1 + P.max_solvent_neighbors);
+}
+
+void d2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+}
+
+// full lists <-> one slot of words
+void save_lists(lima_engine* E, int* dst) {
+    const lima_run_params& P = E->params;
+    const size_t nc = E->NC, ns = E->NS;
+    hipStream_t st = E->stream;
+    d2d(dst, E->cc_cnt.p, nc * 4, st); dst += nc;
+    d2d(dst, E->cc_ids.p, nc * P.max_compound_neighbors * 4, st); dst += nc * P.max_compound_neighbors;
+    d2d(dst, E->cs_cnt.p, nc * 4, st); dst += nc;
+    d2d(dst, E->cs_ids.p, nc * P.max_compound_solvents * 4, st); dst += nc * P.max_compound_solvents;
+    d2d(dst, E->ss_cnt.p, ns * 4, st); dst += ns;
+    d2d(dst, E->ss_ids.p, ns * P.max_solvent_neighbors * 4, st);
+}
+void load_lists(lima_engine* E, const int* src) {
+    const lima_run_params& P = E->params;
+    const size_t nc = E->NC, ns = E->NS;
+    hipStream_t st = E->stream;
+    d2d(E->cc_cnt.p, src, nc * 4, st); src += nc;
+    d2d(E->cc_ids.p, src, nc * P.max_compound_neighbors * 4, st); src += nc * P.max_compound_neighbors;
+    d2d(E->cs_cnt.p, src, nc * 4, st); src += nc;
+    d2d(E->cs_ids.p, src, nc * P.max_compound_solvents * 4, st); src += nc * P.max_compound_solvents;
+    d2d(E->ss_cnt.p, src, ns * 4, st); src += ns;
+    d2d(E->ss_ids.p, src, ns * P.max_solvent_neighbors * 4, st);
+    if (E->NC > 0) nl_work_kernel<<<1, 1024, 0, st>>>(E->nlist_args(E->cur));   // listed work list of these lists
+}
+size_t cc_words(const lima_engine* E) { return (size_t)E->NC * (1 + E->params.max_compound_neighbors); }
+void save_cc(lima_engine* E, int* dst) {
+    d2d(dst, E->cc_cnt.p, (size_t)E->NC * 4, E->stream);
+    d2d(dst + E->NC, E->cc_ids.p, (size_t)E->NC * E->params.max_compound_neighbors * 4, E->stream);
+}
+void load_cc(lima_engine* E, const int* src) {
+    d2d(E->cc_cnt.p, src, (size_t)E->NC * 4, E->stream);
+    d2d(E->cc_ids.p, src + E->NC, (size_t)E->NC * E->params.max_compound_neighbors * 4, E->stream);
+}
+// state (positions of the current parity and the previous positions) <-> a checkpoint
+void save_state(lima_engine* E, float4* dst) {
+    const size_t nch = E->NCH, ns = E->NS;
+    d2d(dst, E->cpos[E->cur].p, nch * 16, E->stream);
+    d2d(dst + nch, E->cprev.p, nch * 16, E->stream);
+    d2d(dst + 2 * nch, E->spos[E->cur].p, ns * 16, E->stream);
+    d2d(dst + 2 * nch + ns, E->sprev.p, ns * 16, E->stream);
+}
+void load_state(lima_engine* E, const float4* src) {
+    const size_t nch = E->NCH, ns = E->NS;
+    d2d(E->cpos[E->cur].p, src, nch * 16, E->stream);
+    d2d(E->cprev.p, src + nch, nch * 16, E->stream);
+    d2d(E->spos[E->cur].p, src + 2 * nch, ns * 16, E->stream);
+    d2d(E->sprev.p, src + 2 * nch + ns, ns * 16, E->stream);
+}
+
+template <typename T> void ensure(DBuf<T>& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+}
+
+// per-particle constants of the adjoint (once per engine)
+void adj_setup(lima_engine* E) {
+    AdjBufs& B = E->adjb;
+    if (B.ready) return;
+    const int N = E->NCH + E->NS;
+    std::vector<float> cfac(std::max(N, 1));
+    std::vector<int> prep(std::max(N, 1));
+    std::vector<float> mass_h((size_t)E->R * kMaxTypes);
+    HIP_TRY(hipMemcpy(mass_h.data(), E->mass.p, mass_h.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<float4> cpos_h(std::max(E->NCH, 1));
+    if (E->NCH) HIP_TRY(hipMemcpy(cpos_h.data(), E->cpos[E->cur].p, (size_t)E->NCH * 16, hipMemcpyDeviceToHost));
+    int ncmax = 1, nsmax = 1;
+    std::vector<int> mask_off(std::max(E->NC, 1), 0);
+    int words = 0;
+    for (int r = 0; r < E->R; r++) {
+        const ReplicaInfo& ri = E->reps_h[r];
+        const double dt = ri.dt;
+        for (int i = ri.chain_begin; i < ri.chain_end; i++) {
+            int t;
+            std::memcpy(&t, &cpos_h[i].w, 4);
+            cfac[i] = (float)(dt / (double)mass_h[r * kMaxTypes + t] * dt);
+            prep[i] = r;
+        }
+        for (int s = ri.solv_begin; s < ri.solv_end; s++) {
+            cfac[E->NCH + s] = (float)(dt / (double)mass_h[r * kMaxTypes] * dt);
+            prep[E->NCH + s] = r;
+        }
+        ncmax = std::max(ncmax, (ri.comp_end - ri.comp_begin + kAdjCompChunk - 1) / kAdjCompChunk);
+        nsmax = std::max(nsmax, (ri.solv_end - ri.solv_begin + kAdjSolvSlice - 1) / kAdjSolvSlice);
+        const int wr = (ri.solv_end - ri.solv_begin + 31) / 32;
+        for (int c = ri.comp_begin; c < ri.comp_end; c++) {
+            mask_off[c] = words;
+            words += wr;
+        }
+    }
+    B.n_cchunks = ncmax;
+    B.n_sslices = nsmax;
+    B.cfac.upload(cfac);
+    B.prep.upload(prep);
+    B.mask_off.upload(mask_off);
+    B.mask.alloc(std::max(words, 1));
+    B.adj.alloc((size_t)3 * std::max(N, 1));
+    B.w.alloc(std::max(N, 1));
+    B.gxc.alloc((size_t)(1 + B.n_sslices) * std::max(E->NCH, 1));
+    B.gxs.alloc((size_t)B.n_cchunks * std::max(E->NS, 1));
+    B.beta.alloc(E->R);
+    B.loss.alloc(E->R);
+    B.gpar.alloc((size_t)E->R * kAdjStride);
+    B.ready = true;
+}
+
+void adj_build_mask(lima_engine* E) {
+    AdjBufs& B = E->adjb;
+    if (E->NC == 0 || E->NS == 0) return;
+    adj_mask_zero_kernel<<<(int)((B.mask.n + 255) / 256), 256, 0, E->stream>>>(B.mask.p, (int)B.mask.n);
+    adj_mask_set_kernel<<<E->NC, 256, 0, E->stream>>>(E->step_args(E->cur), B.mask.p, B.mask_off.p);
+}
+
+// the residuals 2 beta (F - F*) of the frame just stepped into Rs (and the loss when given)
+void adj_residuals(lima_engine* E, float4* rs, const float4* tgt, double* loss, bool solv) {
+    AdjBufs& B = E->adjb;
+    const int nch = E->NCH, ns = E->NS;
+    if (nch > 0)
+        adj_residual_kernel<<<(nch + 255) / 256, 256, 0, E->stream>>>(E->cforce.p, tgt, B.prep.p, B.beta.p, rs, loss, nch);
+    if (ns > 0) {
+        if (solv)
+            adj_residual_kernel<<<(ns + 255) / 256, 256, 0, E->stream>>>(E->sforce.p, tgt ? tgt + nch : nullptr,
+                                                                          B.prep.p + nch, B.beta.p, rs + nch, loss, ns);
+        else
+            HIP_TRY(hipMemsetAsync(rs + nch, 0, (size_t)ns * 16, E->stream));
+    }
+}
+
+// targets of frames [k0, k1) into T (float4, chain then solvent per frame)
+void adj_upload_targets(lima_engine* E, const lima_adjoint_spec& sp, int k0, int k1) {
+    AdjBufs& B = E->adjb;
+    const size_t N = (size_t)E->NCH + E->NS;
+    std::vector<float4> h((size_t)(k1 - k0) * N, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int k = k0; k < k1; k++) {
+        float4* dst = &h[(size_t)(k - k0) * N];
+        if (sp.chain_force_target) {
+            const float* src = sp.chain_force_target + (size_t)k * E->NCH * 3;
+            for (int i = 0; i < E->NCH; i++) dst[i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
+        }
+        if (sp.solvent_force_target) {
+            const float* src = sp.solvent_force_target + (size_t)k * E->NS * 3;
+            for (int i = 0; i < E->NS; i++)
+                dst[E->NCH + i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
+        }
+    }
+    // the previous segment's kernels on the (non-blocking) engine stream may still read T
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    ensure(B.T, h.size());
+    HIP_TRY(hipMemcpy(B.T.p, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
+}
+
+// one step of the trajectory (the engine's step, optionally with FP64 energies)
+void adj_step(lima_engine* E, bool energy) {
+    if (energy) HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
+    E->launch_step(E->cur, energy);
+    E->cur ^= 1;
+    E->step++;
+    E->since_rebuild++;
+}
+
+// one frame of the adjoint recurrence: positions X, residual Rs, energy weights gE
+void adj_frame(lima_engine* E, int j, const float4* X, const float4* rs, const float* gE, bool solv) {
+    AdjBufs& B = E->adjb;
+    const int nch = E->NCH, ns = E->NS, N = nch + ns;
+    float4* a_out = B.adj.p + (size_t)(j % 3) * N;
+    const float4* a1 = B.adj.p + (size_t)((j + 1) % 3) * N;
+    const float4* a2 = B.adj.p + (size_t)((j + 2) % 3) * N;
+    adj_weight_kernel<<<(N + 255) / 256, 256, 0, E->stream>>>(rs, a1, B.cfac.p, B.w.p, N);
+    AdjArgs A{};
+    A.s = E->step_args(E->cur);
+    A.s.cpos = X;
+    A.s.spos = X + nch;
+    A.wc = B.w.p;
+    A.ws = B.w.p + nch;
+    A.gE = gE;
+    A.gxc = B.gxc.p;
+    A.gxs = B.gxs.p;
+    A.gpar = B.gpar.p;
+    A.cs_mask = B.mask.p;
+    A.mask_off = B.mask_off.p;
+    A.n_chain = nch;
+    const bool sv = solv && ns > 0;
+    if (sv) adj_solvent_kernel<<<dim3(E->n_stiles, B.n_cchunks), 256, 0, E->stream>>>(A);
+    if (nch > 0) {
+        if (sv) adj_chain_solvent_kernel<<<dim3(E->NC, B.n_sslices), kCompoundBlock, 0, E->stream>>>(A);
+        adj_compound_kernel<<<E->NC, kCompoundBlock, 0, E->stream>>>(A);
+        if (E->NB > 0) adj_bridge_kernel<<<(E->NB + 63) / 64, 64, 0, E->stream>>>(A);
+    }
+    adj_update_kernel<<<(N + 255) / 256, 256, 0, E->stream>>>(B.gxc.p, nch > 0 ? 1 + (sv ? B.n_sslices : 0) : 0, B.gxs.p,
+                                                              sv ? B.n_cchunks : 0, a1, a2, a_out, nch, ns);
+}
+
+struct SavedRun {
+    int cur;
+    long long step, since, nreb;
+};
+
+int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, double* grad_out) {
+    const lima_run_params& P = E->params;
+    if (P.thermostat != LIMA_THERMOSTAT_NONE)
+        throw LimaError(LIMA_E_UNSUPPORTED, "the adjoint differentiates the plain Verlet step: use LIMA_THERMOSTAT_NONE");
+    const int K = sp.n_frames;
+    if (K < 1) throw LimaError(LIMA_E_ARG, "n_frames must be >= 1");
+    if (!loss_out) throw LimaError(LIMA_E_ARG, "null loss output");
+    const int R = E->R, NCH = E->NCH, NS = E->NS, N = NCH + NS, NC = E->NC;
+    const bool solv = P.enable_solvents && NS > 0;
+    const int iv = P.nlist_interval;
+    const int cpi = sp.checkpoint_interval > 0 ? sp.checkpoint_interval : 20;
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    adj_setup(E);
+    AdjBufs& B = E->adjb;
+
+    // the symmetric-pair terms need symmetric compound lists (the engine builds them so)
+    if (NC > 0) {
+        std::vector<int> cnt(NC), ids((size_t)NC * P.max_compound_neighbors);
+        HIP_TRY(hipMemcpy(cnt.data(), E->cc_cnt.p, NC * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ids.data(), E->cc_ids.p, ids.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<std::vector<int>> sets(NC);
+        for (int c = 0; c < NC; c++) sets[c].assign(&ids[(size_t)c * P.max_compound_neighbors], &ids[(size_t)c * P.max_compound_neighbors] + cnt[c]);
+        for (int c = 0; c < NC; c++)
+            for (int nb : sets[c])
+                if (nb < 0 || nb >= NC || std::find(sets[nb].begin(), sets[nb].end(), c) == sets[nb].end())
+                    throw LimaError(LIMA_E_ARG, "the adjoint needs symmetric compound neighbour lists");
+    }
+
+    const SavedRun saved{E->cur, E->step, E->since_rebuild, E->n_rebuilds};
+    const long long since0 = E->since_rebuild;
+    auto rebuild_after = [&](int j) { return iv > 0 && (since0 + j + 1) % iv == 0; };
+    // segment starts
+    std::vector<int> starts{0};
+    for (int j = 0; j + 1 < K; j++) {
+        const bool boundary = iv > 0 ? rebuild_after(j) : true;
+        if (boundary && j + 1 - starts.back() >= cpi) starts.push_back(j + 1);
+    }
+    const int nseg = (int)starts.size();
+    int max_len = 0, max_inner = 0;
+    for (int s = 0; s < nseg; s++) {
+        const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
+        max_len = std::max(max_len, k1 - k0);
+        int inner = 0;
+        for (int j = k0; j + 1 < k1; j++) inner += rebuild_after(j) ? 1 : 0;
+        max_inner = std::max(max_inner, inner);
+    }
+    const size_t LW = std::max<size_t>(list_words(E), 1), CW = std::max<size_t>(cc_words(E), 1);
+    ensure(B.ck, (size_t)nseg * 2 * std::max(N, 1));
+    ensure(B.ck_cc, (size_t)nseg * CW);
+    ensure(B.lists, (size_t)(1 + max_inner) * LW);
+    ensure(B.X, (size_t)max_len * std::max(N, 1));
+    ensure(B.Rs, (size_t)max_len * std::max(N, 1));
+    ensure(B.efr, (size_t)K * R * kEnergyTerms);
+    ensure(B.gE, (size_t)K * R);
+
+    std::vector<double> beta(R, 0.0), alpha(R, 0.0);
+    for (int r = 0; r < R; r++) {
+        if (sp.force_weight) beta[r] = sp.force_weight[r];
+        if (sp.energy_weight) alpha[r] = sp.energy_weight[r];
+    }
+    HIP_TRY(hipMemcpy(B.beta.p, beta.data(), R * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(B.loss.p, 0, R * 8, E->stream));
+    const bool has_tgt = sp.chain_force_target || sp.solvent_force_target;
+
+    // ---- forward pass: checkpoints, energies, force loss ----
+    save_lists(E, B.lists.p);
+    for (int s = 0; s < nseg; s++) {
+        const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
+        save_state(E, B.ck.p + (size_t)s * 2 * N);
+        if (has_tgt) adj_upload_targets(E, sp, k0, k1);
+        for (int j = k0; j < k1; j++) {
+            adj_step(E, true);
+            d2d(B.efr.p + (size_t)j * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8, E->stream);
+            adj_residuals(E, B.Rs.p, has_tgt ? B.T.p + (size_t)(j - k0) * N : nullptr, B.loss.p, solv);
+            if (j + 1 < K && rebuild_after(j)) {
+                if (s + 1 < nseg && starts[s + 1] == j + 1) save_cc(E, B.ck_cc.p + (size_t)(s + 1) * CW);
+                E->launch_rebuild(E->cur);
+                E->since_rebuild = 0;
+            }
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    int st = E->check_errors();
+    std::vector<double> efr((size_t)K * R * kEnergyTerms), floss(R);
+    HIP_TRY(hipMemcpy(efr.data(), B.efr.p, efr.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(floss.data(), B.loss.p, R * 8, hipMemcpyDeviceToHost));
+    std::vector<float> gE((size_t)K * R);
+    for (int r = 0; r < R; r++) loss_out[r] = floss[r];
+    for (int k = 0; k < K; k++)
+        for (int r = 0; r < R; r++) {
+            const double* v = &efr[((size_t)k * R + r) * kEnergyTerms];
+            const double pot = v[kE_bond] + v[kE_angle] + v[kE_dih] +
+                               0.5 * (v[kE_intra] + v[kE_inter] + v[kE_cs] + v[kE_sc] + v[kE_ss]);
+            const double tgt = sp.energy_target ? sp.energy_target[(size_t)k * R + r] : 0.0;
+            loss_out[r] += alpha[r] * (pot - tgt) * (pot - tgt);
+            gE[(size_t)k * R + r] = (float)(2.0 * alpha[r] * (pot - tgt));
+        }
+
+    // ---- backward pass ----
+    if (st == LIMA_OK && grad_out) {
+        HIP_TRY(hipMemcpy(B.gE.p, gE.data(), gE.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(B.adj.p, 0, (size_t)3 * N * 16, E->stream));
+        HIP_TRY(hipMemsetAsync(B.gpar.p, 0, (size_t)R * kAdjStride * 8, E->stream));
+        for (int s = nseg - 1; s >= 0; s--) {
+            const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
+            load_state(E, B.ck.p + (size_t)s * 2 * N);
+            if (k0 == 0 || iv <= 0) {
+                load_lists(E, B.lists.p);
+                E->since_rebuild = iv > 0 ? (since0 + k0) % iv : 0;
+            } else {
+                load_cc(E, B.ck_cc.p + (size_t)s * CW);
+                E->launch_rebuild(E->cur);
+                E->since_rebuild = 0;
+            }
+            if (has_tgt) adj_upload_targets(E, sp, k0, k1);
+            // recompute the segment
+            std::vector<int> period_end;
+            for (int j = k0; j < k1; j++) {
+                float4* X = B.X.p + (size_t)(j - k0) * N;
+                d2d(X, E->cpos[E->cur].p, (size_t)NCH * 16, E->stream);
+                d2d(X + NCH, E->spos[E->cur].p, (size_t)NS * 16, E->stream);
+                adj_step(E, false);
+                adj_residuals(E, B.Rs.p + (size_t)(j - k0) * N, has_tgt ? B.T.p + (size_t)(j - k0) * N : nullptr,
+                              nullptr, solv);
+                if (j + 1 < k1 && rebuild_after(j)) {
+                    save_lists(E, B.lists.p + (size_t)(1 + period_end.size()) * LW);
+                    period_end.push_back(j);
+                    E->launch_rebuild(E->cur);
+                    E->since_rebuild = 0;
+                }
+            }
+            // reverse over the segment's frames
+            int q = (int)period_end.size();
+            bool mask_dirty = true;
+            for (int j = k1 - 1; j >= k0; j--) {
+                if (q > 0 && j == period_end[q - 1]) {
+                    load_lists(E, B.lists.p + (size_t)q * LW);
+                    q--;
+                    mask_dirty = true;
+                }
+                if (mask_dirty) {
+                    adj_build_mask(E);
+                    mask_dirty = false;
+                }
+                adj_frame(E, j, B.X.p + (size_t)(j - k0) * N, B.Rs.p + (size_t)(j - k0) * N, B.gE.p + (size_t)j * R, solv);
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        st = E->check_errors();
+        std::vector<double> gp((size_t)R * kAdjStride);
+        HIP_TRY(hipMemcpy(gp.data(), B.gpar.p, gp.size() * 8, hipMemcpyDeviceToHost));
+        for (int r = 0; r < R; r++) {
+            const double* g = &gp[(size_t)r * kAdjStride];
+            double* o = grad_out + (size_t)r * LIMA_N_PARAMETERS;
+            const lima_particle_params* pp = E->ff_h[r].particle_parameters;
+            for (int t = 0; t < kMaxTypes; t++) {
+                o[t] = g[t];
+                const double eps = pp[t].epsilon;
+                o[kMaxTypes + t] = 12.0 * g[kMaxTypes + t] + (eps > 0 ? g[2 * kMaxTypes + t] / (2.0 * eps) : 0.0);
+            }
+            for (int k = 0; k < 4; k++) o[2 * kMaxTypes + k] = g[kAdjTypeBins + k];
+        }
+    }
+
+    // ---- leave the engine as it was ----
+    E->cur = saved.cur;
+    load_state(E, B.ck.p);
+    load_lists(E, B.lists.p);
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->step = saved.step;
+    E->since_rebuild = saved.since;
+    E->n_rebuilds = saved.nreb;
+    return st;
+}
+
+// forces and potential energies of frames 0..K-1 from the current state
+int record_frames(lima_engine* E, int K, float* cf, float* sf, double* pot) {
+    if (K < 1) throw LimaError(LIMA_E_ARG, "n_frames must be >= 1");
+    const int R = E->R, NCH = E->NCH, NS = E->NS, N = NCH + NS;
+    const int iv = E->params.nlist_interval;
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    AdjBufs& B = E->adjb;
+    ensure(B.ck, (size_t)2 * std::max(N, 1));
+    ensure(B.lists, std::max<size_t>(list_words(E), 1));
+    const SavedRun saved{E->cur, E->step, E->since_rebuild, E->n_rebuilds};
+    save_state(E, B.ck.p);
+    save_lists(E, B.lists.p);
+    std::vector<float4> hc(std::max(NCH, 1)), hs(std::max(NS, 1));
+    std::vector<double> e((size_t)R * kEnergyTerms);
+    int st = LIMA_OK;
+    for (int j = 0; j < K && st == LIMA_OK; j++) {
+        adj_step(E, true);
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        st = E->check_errors();
+        if (cf && NCH) {
+            HIP_TRY(hipMemcpy(hc.data(), E->cforce.p, (size_t)NCH * 16, hipMemcpyDeviceToHost));
+            for (int i = 0; i < NCH; i++) {
+                float* o = cf + ((size_t)j * NCH + i) * 3;
+                o[0] = hc[i].x; o[1] = hc[i].y; o[2] = hc[i].z;
+            }
+        }
+        if (sf && NS) {
+            const bool solv = E->params.enable_solvents;
+            if (solv) HIP_TRY(hipMemcpy(hs.data(), E->sforce.p, (size_t)NS * 16, hipMemcpyDeviceToHost));
+            for (int i = 0; i < NS; i++) {
+                float* o = sf + ((size_t)j * NS + i) * 3;
+                o[0] = solv ? hs[i].x : 0.f; o[1] = solv ? hs[i].y : 0.f; o[2] = solv ? hs[i].z : 0.f;
+            }
+        }
+        if (pot) {
+            HIP_TRY(hipMemcpy(e.data(), E->energy.p, e.size() * 8, hipMemcpyDeviceToHost));
+            for (int r = 0; r < R; r++) {
+                const double* v = &e[(size_t)r * kEnergyTerms];
+                pot[(size_t)j * R + r] = v[kE_bond] + v[kE_angle] + v[kE_dih] +
+                                         0.5 * (v[kE_intra] + v[kE_inter] + v[kE_cs] + v[kE_sc] + v[kE_ss]);
+            }
+        }
+        if (iv > 0 && E->since_rebuild >= iv) {
+            E->launch_rebuild(E->cur);
+            E->since_rebuild = 0;
+        }
+    }
+    E->cur = saved.cur;
+    load_state(E, B.ck.p);
+    load_lists(E, B.lists.p);
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->step = saved.step;
+    E->since_rebuild = saved.since;
+    E->n_rebuilds = saved.nreb;
+    return st;
+}
+
+// parameter vector of one system <-> force field and shared bonded constants
+void get_parameters(lima_engine* E, int r, double* o) {
+    const lima_particle_params* pp = E->ff_h[r].particle_parameters;
+    for (int t = 0; t < kMaxTypes; t++) {
+        o[t] = pp[t].sigma;
+        o[kMaxTypes + t] = pp[t].epsilon;
+    }
+    const ReplicaInfo& ri = E->reps_h[r];
+    const int b0 = E->bond_off_h[ri.comp_begin], b1 = E->bond_off_h[ri.comp_end];
+    const int a0 = E->angle_off_h[ri.comp_begin], a1 = E->angle_off_h[ri.comp_end];
+    o[64] = o[65] = o[66] = o[67] = 0.0;
+    const BondD* bd = b1 > b0 ? &E->bonds_h[b0] : nullptr;
+    const AngleD* ad = a1 > a0 ? &E->angles_h[a0] : nullptr;
+    for (int b = 0; b < E->NB && (!bd || !ad); b++) {
+        if (E->br_rep_h[b] != r) continue;
+        if (!bd && E->brb_off_h[b + 1] > E->brb_off_h[b]) bd = &E->brb_h[E->brb_off_h[b]];
+        if (!ad && E->bra_off_h[b + 1] > E->bra_off_h[b]) ad = &E->bra_h[E->bra_off_h[b]];
+    }
+    if (bd) { o[64] = bd->kb; o[65] = bd->b0; }
+    if (ad) { o[66] = ad->k; o[67] = ad->t0; }
+}
+
+void set_parameters(lima_engine* E, int r, const double* v) {
+    for (int i = 0; i < LIMA_N_PARAMETERS; i++)
+        if (!std::isfinite(v[i])) throw LimaError(LIMA_E_ARG, "parameters must be finite");
+    for (int t = 0; t < kMaxTypes; t++)
+        if (v[t] <= 0 || v[kMaxTypes + t] < 0) throw LimaError(LIMA_E_ARG, "sigma must be > 0 and epsilon >= 0");
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    lima_particle_params* pp = E->ff_h[r].particle_parameters;
+    for (int t = 0; t < kMaxTypes; t++) {
+        pp[t].sigma = (float)v[t];
+        pp[t].epsilon = (float)v[kMaxTypes + t];
+    }
+    std::vector<float> mass(kMaxTypes);
+    std::vector<float4> tsc(kMaxTypes), tcs(kMaxTypes), tinter(1024), tintra(1024), tss(1);
+    fill_tables(pp, mass.data(), tsc.data(), tcs.data(), tinter.data(), tintra.data(), tss.data());
+    HIP_TRY(hipMemcpy(E->tab_sc.p + r * kMaxTypes, tsc.data(), kMaxTypes * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->tab_cs.p + r * kMaxTypes, tcs.data(), kMaxTypes * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->tab_inter.p + (size_t)r * 1024, tinter.data(), 1024 * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->tab_intra.p + (size_t)r * 1024, tintra.data(), 1024 * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->tab_ss.p + r, tss.data(), 16, hipMemcpyHostToDevice));
+    const ReplicaInfo& ri = E->reps_h[r];
+    for (int b = E->bond_off_h[ri.comp_begin]; b < E->bond_off_h[ri.comp_end]; b++) {
+        E->bonds_h[b].kb = (float)v[64];
+        E->bonds_h[b].b0 = (float)v[65];
+    }
+    for (int a = E->angle_off_h[ri.comp_begin]; a < E->angle_off_h[ri.comp_end]; a++) {
+        E->angles_h[a].k = (float)v[66];
+        E->angles_h[a].t0 = (float)v[67];
+    }
+    for (int b = 0; b < E->NB; b++) {
+        if (E->br_rep_h[b] != r) continue;
+        for (int t = E->brb_off_h[b]; t < E->brb_off_h[b + 1]; t++) {
+            E->brb_h[t].kb = (float)v[64];
+            E->brb_h[t].b0 = (float)v[65];
+        }
+        for (int t = E->bra_off_h[b]; t < E->bra_off_h[b + 1]; t++) {
+            E->bra_h[t].k = (float)v[66];
+            E->bra_h[t].t0 = (float)v[67];
+        }
+    }
+    auto put = [](auto& dbuf, const auto& host) {
+        if (!host.empty()) HIP_TRY(hipMemcpy(dbuf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice));
+    };
+    put(E->bonds, E->bonds_h);
+    put(E->angles, E->angles_h);
+    put(E->brb, E->brb_h);
+    put(E->bra, E->bra_h);
 }
 
 thread_local std::string g_last_error;
@@ -1236,6 +1809,33 @@ int lima_autotune(lima_engine* E, int32_t reps) {
         if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
         return E->check_errors();
     });
+}
+
+int lima_get_parameters(lima_engine* E, int32_t system, double* params) {
+    return guarded(E, [&] {
+        if (!params || system < 0 || system >= E->R) throw LimaError(LIMA_E_ARG, "system index out of range or null output");
+        get_parameters(E, system, params);
+        return LIMA_OK;
+    });
+}
+
+int lima_set_parameters(lima_engine* E, int32_t system, const double* params) {
+    return guarded(E, [&] {
+        if (!params || system < 0 || system >= E->R) throw LimaError(LIMA_E_ARG, "system index out of range or null input");
+        set_parameters(E, system, params);
+        return LIMA_OK;
+    });
+}
+
+int lima_adjoint_run(lima_engine* E, const lima_adjoint_spec* spec, double* loss, double* grad) {
+    return guarded(E, [&] {
+        if (!spec) throw LimaError(LIMA_E_ARG, "null spec");
+        return adjoint_run(E, *spec, loss, grad);
+    });
+}
+
+int lima_record_frames(lima_engine* E, int32_t n_frames, float* chain_forces, float* solvent_forces, double* potential) {
+    return guarded(E, [&] { return record_frames(E, n_frames, chain_forces, solvent_forces, potential); });
 }
 
 }  // extern "C"

@@ -60,7 +60,7 @@ struct StepArgs {
     const int4* stiles;           // (replica, begin, end, tile index) per solvent-integrate tile
     // type tables per replica
     const float* mass;            // [R*32]
-    // pair parameters (sigma^2, 24 eps, -48 eps, 0) with each call site's mixing rule
+    // pair parameters (sigma^2, 24 eps, -48 eps, sigma) with each call site's mixing rule
     const float4* tab_sc;         // [R*32] solvent <- chain type, arithmetic eps
     const float4* tab_cs;         // [R*32] chain type <- solvent, geometric eps
     const float4* tab_inter;      // [R*1024] arithmetic

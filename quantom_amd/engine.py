@@ -17,7 +17,8 @@ import os
 
 import numpy as np
 
-from .abi import (CEnergies, CRunParams, CSystem, LimaSystem, RunParams, STATUS_NAMES, LIMA_OK)
+from .abi import (CAdjointSpec, CEnergies, CRunParams, CSystem, LimaSystem, N_PARAMETERS, RunParams, STATUS_NAMES,
+                  LIMA_OK)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblima_hip.so")
@@ -28,7 +29,7 @@ EXPORTS = (
     "lima_step_async", "lima_sync", "lima_rebuild_neighbors", "lima_get_state", "lima_set_state",
     "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
     "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_get_config",
-    "lima_autotune",
+    "lima_autotune", "lima_get_parameters", "lima_set_parameters", "lima_adjoint_run", "lima_record_frames",
 )
 
 
@@ -70,12 +71,21 @@ def load_library() -> C.CDLL:
     L.lima_get_timings.argtypes = [vp, C.POINTER(C.c_double)]
     L.lima_get_config.argtypes = [vp, ip]
     L.lima_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    dp = C.POINTER(C.c_double)
+    L.lima_get_parameters.argtypes = [vp, C.c_int32, dp]
+    L.lima_set_parameters.argtypes = [vp, C.c_int32, dp]
+    L.lima_adjoint_run.argtypes = [vp, C.POINTER(CAdjointSpec), dp, dp]
+    L.lima_record_frames.argtypes = [vp, C.c_int32, fp, fp, dp]
     _lib = L
     return L
 
 
 def _f(a):
     return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
 def _i(a):
@@ -247,6 +257,55 @@ class Engine:
     def autotune(self, trial_steps: int = 20) -> dict:
         self._check(self.L.lima_autotune(self.h, trial_steps))
         return self.config()
+
+
+    # ---- parameters and reverse-mode gradients ----
+    def get_parameters(self, system: int = 0) -> np.ndarray:
+        """[sigma(32), epsilon(32), kb, b0, k_theta, theta_0] of one system."""
+        v = np.zeros(N_PARAMETERS, np.float64)
+        self._check(self.L.lima_get_parameters(self.h, system, _d(v)))
+        return v
+
+    def set_parameters(self, values, system: int = 0):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (N_PARAMETERS,):
+            raise ValueError(f"expected {N_PARAMETERS} parameters")
+        self._check(self.L.lima_set_parameters(self.h, system, _d(v)))
+
+    def record_frames(self, n_frames: int):
+        """Forces and potential energies of frames 0..K-1 from the current state (state unchanged)."""
+        sim = self.simulation
+        cf = np.zeros((n_frames, sim.n_chain, 3), np.float32)
+        sf = np.zeros((n_frames, sim.n_solvents, 3), np.float32)
+        pot = np.zeros((n_frames, len(sim.systems)), np.float64)
+        self._check(self.L.lima_record_frames(self.h, n_frames, _f(cf), _f(sf), _d(pot)))
+        return cf, sf, pot
+
+    def adjoint(self, n_frames: int, energy_weight=None, force_weight=None, energy_target=None,
+                chain_force_target=None, solvent_force_target=None, checkpoint_interval: int = 20,
+                gradient: bool = True):
+        """Loss per system and, with `gradient`, dLoss/dparameters [n_systems, 68] (state unchanged)."""
+        sim = self.simulation
+        R = len(sim.systems)
+
+        def arr(v, shape, dt):
+            if v is None:
+                return None
+            a = np.ascontiguousarray(v, dtype=dt)
+            if a.size != int(np.prod(shape)):
+                raise ValueError(f"expected {shape} values")
+            return a.reshape(shape)
+
+        keep = [arr(energy_weight, (R,), np.float64), arr(force_weight, (R,), np.float64),
+                arr(energy_target, (n_frames, R), np.float64),
+                arr(chain_force_target, (n_frames, sim.n_chain, 3), np.float32),
+                arr(solvent_force_target, (n_frames, sim.n_solvents, 3), np.float32)]
+        spec = CAdjointSpec(n_frames, checkpoint_interval, _d(keep[0]), _d(keep[1]), _d(keep[2]), _f(keep[3]),
+                            _f(keep[4]))
+        loss = np.zeros(R, np.float64)
+        grad = np.zeros((R, N_PARAMETERS), np.float64) if gradient else None
+        self._check(self.L.lima_adjoint_run(self.h, C.byref(spec), _d(loss), _d(grad)))
+        return (loss, grad) if gradient else loss
 
 
 def version() -> str:

@@ -200,7 +200,9 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
     ff = make_forcefield(rng, n_types) if forcefield is None else forcefield.astype(np.float32)
     L = float(box_len)
     n_beads_chain = compounds_per_chain * compound_size
-    confine = 0.38 * L
+    # every compound stays well inside half a box of its particle 0, against which
+    # its positions are re-imaged: 0.38 L, less a margin in boxes under 5 nm
+    confine = 0.38 * L - max(0.0, 0.6 - 0.12 * L)
     walk = (_walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size) if n_chains > 0
             else np.zeros((0, 3)))
     n_chain = len(walk)
@@ -367,7 +369,7 @@ def large() -> LimaSystem:
 
 
 def replicas(n: int = 256, seed_config: int = 3) -> list[LimaSystem]:
-    """Uneven replicas: chain 128-1024 (1-8 compounds), solvent 500-6000, box from density 33 (3-6 nm)."""
+    """Uneven replicas: chain 128-1024 (1-8 compounds), solvent 500-6000, box from density 33 (3.6-6 nm)."""
     out = []
     for r in range(n):
         seed = SEED_BASE + 1000 * seed_config + r
@@ -376,7 +378,7 @@ def replicas(n: int = 256, seed_config: int = 3) -> list[LimaSystem]:
         nsol = int(rng.integers(500, 6001))
         T0 = float(rng.uniform(280.0, 320.0))
         L = ((ncomp * COMPOUND_SIZE + nsol) / 33.0) ** (1.0 / 3.0)
-        L = float(min(6.0, max(3.0, L)))
+        L = float(min(6.0, max(3.6, L)))
         cap = int(0.8 * 33.0 * L ** 3) - ncomp * COMPOUND_SIZE * 3
         nsol = max(100, min(nsol, cap))
         out.append(build_system(seed, 1, ncomp, L, 16, nsol, T0=T0))

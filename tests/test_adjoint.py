@@ -1,0 +1,315 @@
+"""Reverse-mode gradients of trajectory losses (lima_adjoint_run) and the torch node.
+
+The reference has no differentiable path; the bar is the FP64 oracle: the loss
+of the same frames evaluated by the oracle, and central finite differences of
+that loss with respect to every parameter the gradient reports.  The frames
+cross a neighbour-list rebuild, and the checkpoint interval splits them into
+segments that the backward pass recomputes.
+"""
+import numpy as np
+import pytest
+import torch
+
+from quantom_amd.abi import N_PARAMETERS, THERMOSTAT_NONE, RunParams
+
+K_FRAMES = 14            # crosses the rebuild after frame 9
+PARAMS = RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE)
+
+
+def with_params(sysm, v):
+    """A copy of the system with parameter vector v (force field and shared bonded constants)."""
+    s = sysm.copy()
+    ff = s.forcefield.copy()
+    ff[:, 1] = v[:32]
+    ff[:, 2] = v[32:64]
+    s.forcefield = ff
+    for arr in (s.bonds, s.bridge_bonds):
+        arr["kb"] = v[64]
+        arr["b0"] = v[65]
+    for arr in (s.angles, s.bridge_angles):
+        arr["k_theta"] = v[66]
+        arr["theta_0"] = v[67]
+    return s
+
+
+def base_params(sysm):
+    v = np.zeros(N_PARAMETERS)
+    v[:32] = sysm.forcefield[:, 1]
+    v[32:64] = sysm.forcefield[:, 2]
+    v[64], v[65] = sysm.bonds["kb"][0], sysm.bonds["b0"][0]
+    v[66], v[67] = sysm.angles["k_theta"][0], sysm.angles["theta_0"][0]
+    return v
+
+
+def oracle_frames(sysm, K):
+    from oracle.oracle import Oracle
+    o = Oracle(sysm, PARAMS, precision=64)
+    fc, fs, pot = [], [], []
+    for k in range(K):
+        cf, sf, st = o.compute_forces()
+        assert st == 0
+        fc.append(cf)
+        fs.append(sf)
+        pot.append(o.energies()["potential"])
+        if k + 1 < K:
+            assert o.step(1) == 0
+    return np.array(fc), np.array(fs), np.array(pot)
+
+
+def oracle_loss(sysm, K, alpha, beta, et, fct, fst):
+    fc, fs, pot = oracle_frames(sysm, K)
+    return float(alpha * ((pot - et) ** 2).sum() + beta * (((fc - fct) ** 2).sum() + ((fs - fst) ** 2).sum()))
+
+
+@pytest.fixture(scope="module")
+def problem(tiny_system):
+    """Targets from the FP64 oracle at perturbed parameters; weights normalise both terms."""
+    v0 = base_params(tiny_system)
+    vt = v0.copy()
+    rng = np.random.default_rng(7)
+    vt[:5] *= 1.0 + 0.03 * rng.uniform(-1, 1, 5)
+    vt[32:37] *= 1.0 + 0.08 * rng.uniform(-1, 1, 5)
+    vt[64] *= 1.05
+    vt[65] *= 0.99
+    vt[66] *= 0.95
+    vt[67] += 0.01
+    fct, fst, et = oracle_frames(with_params(tiny_system, vt.astype(np.float32).astype(np.float64)), K_FRAMES)
+    bfc, bfs, bpot = oracle_frames(tiny_system, K_FRAMES)
+    beta = 1.0 / float((fct ** 2).sum() + (fst ** 2).sum())
+    # energy weight chosen so that both terms contribute equally at the base parameters
+    alpha = beta * float(((bfc - fct) ** 2).sum() + ((bfs - fst) ** 2).sum()) / float(((bpot - et) ** 2).sum())
+    return dict(v0=v0, et=et, fct=fct.astype(np.float32), fst=fst.astype(np.float32), alpha=alpha, beta=beta)
+
+
+def used_types(sysm):
+    return sorted(set(int(t) for t in sysm.chain_atom_types) | {0})
+
+
+# ---------------------------------------------------------------- CPU: host plumbing
+class _FakeEngine:
+    """Stands in for Engine in the autograd node: fixed loss and gradient per system."""
+
+    class _Sim:
+        def __init__(self, R):
+            self.systems = [None] * R
+
+    def __init__(self, R):
+        self.simulation = self._Sim(R)
+        self.params_set = {}
+        self.grad = np.arange(R * N_PARAMETERS, dtype=np.float64).reshape(R, N_PARAMETERS) * 0.5
+        self.calls = []
+
+    def set_parameters(self, v, r):
+        self.params_set[r] = np.array(v)
+
+    def adjoint(self, n_frames, *args, checkpoint_interval=20, gradient=True):
+        self.calls.append((n_frames, checkpoint_interval, gradient))
+        loss = np.array([float(v.sum()) for _, v in sorted(self.params_set.items())])
+        return (loss, self.grad.copy()) if gradient else loss
+
+
+def test_autograd_node_scales_stored_gradient():
+    from quantom_amd.adjoint import trajectory_loss
+    eng = _FakeEngine(2)
+    p = torch.ones(2, N_PARAMETERS, dtype=torch.float64, requires_grad=True)
+    loss = trajectory_loss(eng, p, 5, checkpoint_interval=7)
+    assert loss.shape == (2,)
+    assert eng.calls == [(5, 7, True)]
+    (loss * torch.tensor([2.0, -3.0], dtype=torch.float64)).sum().backward()
+    expect = torch.from_numpy(eng.grad) * torch.tensor([[2.0], [-3.0]], dtype=torch.float64)
+    assert torch.equal(p.grad, expect)
+    np.testing.assert_array_equal(eng.params_set[1], np.ones(N_PARAMETERS))
+
+
+def test_autograd_node_single_system_and_no_grad():
+    from quantom_amd.adjoint import trajectory_loss
+    eng = _FakeEngine(1)
+    p = torch.full((N_PARAMETERS,), 2.0, dtype=torch.float64, requires_grad=True)
+    loss = trajectory_loss(eng, p, 3)
+    assert loss.dim() == 0 and float(loss) == 2.0 * N_PARAMETERS
+    loss.backward()
+    assert torch.equal(p.grad, torch.from_numpy(eng.grad[0]))
+    with torch.no_grad():
+        trajectory_loss(eng, p.detach(), 3)
+    assert eng.calls[-1][2] is False
+
+
+def test_parameter_vector_round_trip_on_the_system(tiny_system):
+    v = base_params(tiny_system)
+    s = with_params(tiny_system, v)
+    np.testing.assert_array_equal(base_params(s), v.astype(np.float32).astype(np.float64))
+    assert np.all(s.bonds["kb"] == np.float32(v[64])) and np.all(s.bridge_angles["theta_0"] == np.float32(v[67]))
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.mark.gpu
+def test_parameters_get_set(tiny_system):
+    from quantom_amd.engine import Engine, LimaStatusError
+    e = Engine([tiny_system], PARAMS)
+    v = e.get_parameters(0)
+    np.testing.assert_array_equal(v, base_params(tiny_system).astype(np.float32).astype(np.float64))
+    w = v.copy()
+    w[3] *= 1.1
+    w[66] = 3.0e5
+    e.set_parameters(w, 0)
+    np.testing.assert_allclose(e.get_parameters(0), w.astype(np.float32), rtol=0, atol=0)
+    # the engine now steps with the new parameters: same forces as a fresh engine built with them
+    f_set = e.compute_forces()
+    f_new = Engine([with_params(tiny_system, w)], PARAMS).compute_forces()
+    np.testing.assert_array_equal(f_set[0], f_new[0])
+    np.testing.assert_array_equal(f_set[1], f_new[1])
+    bad = w.copy()
+    bad[0] = -1.0
+    with pytest.raises(LimaStatusError):
+        e.set_parameters(bad, 0)
+    e.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_loss_matches_fp64_oracle(tiny_system, problem):
+    from quantom_amd.engine import Engine
+    P = problem
+    e = Engine([tiny_system], PARAMS)
+    loss = e.adjoint(K_FRAMES, [P["alpha"]], [P["beta"]], P["et"][:, None], P["fct"], P["fst"], gradient=False)
+    ref = oracle_loss(tiny_system, K_FRAMES, P["alpha"], P["beta"], P["et"], P["fct"], P["fst"])
+    print("loss gpu", loss[0], "oracle fp64", ref)
+    assert abs(loss[0] - ref) <= 1e-4 * abs(ref)
+    e.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_matches_fd_of_fp64_oracle(tiny_system, problem):
+    """Every used sigma/epsilon, the four bonded constants, and two unused types (exact zeros)."""
+    from quantom_amd.engine import Engine
+    P = problem
+    e = Engine([tiny_system], PARAMS)
+    loss, grad = e.adjoint(K_FRAMES, [P["alpha"]], [P["beta"]], P["et"][:, None], P["fct"], P["fst"],
+                           checkpoint_interval=10)
+    g = grad[0]
+    v0 = P["v0"]
+    idx = []
+    for t in used_types(tiny_system):
+        idx += [t, 32 + t]
+    idx += [64, 65, 66, 67]
+    fd = {}
+    for i in idx:
+        h = (1e-4 if i != 67 else 1e-3) * max(abs(v0[i]), 1e-3)
+        vp, vm = v0.copy(), v0.copy()
+        vp[i] = np.float32(v0[i] + h)
+        vm[i] = np.float32(v0[i] - h)
+        lp = oracle_loss(with_params(tiny_system, vp), K_FRAMES, P["alpha"], P["beta"], P["et"], P["fct"], P["fst"])
+        lm = oracle_loss(with_params(tiny_system, vm), K_FRAMES, P["alpha"], P["beta"], P["et"], P["fct"], P["fst"])
+        fd[i] = (lp - lm) / (float(vp[i]) - float(vm[i]))
+    scale = max(abs(x) for x in fd.values())
+    rows = []
+    for i in idx:
+        rows.append((i, g[i], fd[i], abs(g[i] - fd[i]) / max(abs(fd[i]), 1e-30)))
+        print(f"param {i:2d}  adjoint {g[i]: .6e}  fd {fd[i]: .6e}  rel {rows[-1][3]:.2e}")
+    for i, ga, gf, rel in rows:
+        assert abs(ga - gf) <= 2e-3 * abs(gf) + 1e-4 * scale, f"parameter {i}: adjoint {ga} fd {gf}"
+    unused = [t for t in range(1, 32) if t not in used_types(tiny_system)][:2]
+    for t in unused:
+        assert g[t] == 0.0 and g[32 + t] == 0.0
+    e.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_checkpoint_interval_does_not_change_result(tiny_system, problem):
+    from quantom_amd.engine import Engine
+    P = problem
+    e = Engine([tiny_system], PARAMS)
+    args = (K_FRAMES, [P["alpha"]], [P["beta"]], P["et"][:, None], P["fct"], P["fst"])
+    l1, g1 = e.adjoint(*args, checkpoint_interval=10)
+    l2, g2 = e.adjoint(*args, checkpoint_interval=1000)
+    l3, g3 = e.adjoint(*args, checkpoint_interval=10)
+    np.testing.assert_allclose(l1, l2, rtol=1e-12)
+    np.testing.assert_allclose(g1, g2, rtol=1e-5, atol=1e-6 * np.abs(g1).max())
+    np.testing.assert_allclose(g1, g3, rtol=1e-5, atol=1e-6 * np.abs(g1).max())
+    e.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_leaves_engine_unchanged(tiny_system, problem):
+    from quantom_amd.engine import Engine
+    P = problem
+    e = Engine([tiny_system], PARAMS)
+    e.step(3)
+    before = e.get_state()
+    lists_before = e.neighbors()
+    e.adjoint(K_FRAMES, [P["alpha"]], [P["beta"]], None, P["fct"], P["fst"], checkpoint_interval=10)
+    after = e.get_state()
+    for a, b in zip(before, after):
+        np.testing.assert_array_equal(a, b)
+    for la, lb in zip(lists_before, e.neighbors()):
+        assert [list(x) for x in la] == [list(x) for x in lb]
+    e.step(12)
+    ref = Engine([tiny_system], PARAMS)
+    ref.step(15)
+    for a, b in zip(e.get_state(), ref.get_state()):
+        np.testing.assert_array_equal(a, b)
+    e.close()
+    ref.close()
+
+
+@pytest.mark.gpu
+def test_record_frames_and_zero_loss_at_own_targets(tiny_system):
+    from quantom_amd.engine import Engine
+    e = Engine([tiny_system], PARAMS)
+    fc, fs, pot = e.record_frames(K_FRAMES)
+    cf0, sf0 = e.compute_forces()
+    np.testing.assert_array_equal(fc[0], cf0)
+    np.testing.assert_array_equal(fs[0], sf0)
+    ofc, ofs, opot = oracle_frames(tiny_system, K_FRAMES)
+    np.testing.assert_allclose(pot[:, 0], opot, rtol=1e-5)
+    den = np.sqrt((ofc ** 2).sum() + (ofs ** 2).sum())
+    assert np.sqrt(((fc - ofc) ** 2).sum() + ((fs - ofs) ** 2).sum()) / den < 1e-4
+    loss, grad = e.adjoint(K_FRAMES, [1.0], [1.0], pot, fc, fs)
+    assert loss[0] <= 1e-12 * (pot ** 2).sum()
+    assert np.all(np.abs(grad) <= 1e-6 * (1.0 + np.abs(grad).max()))
+    e.close()
+
+
+@pytest.mark.gpu
+def test_batched_adjoint_equals_individual(tiny_system, problem):
+    from quantom_amd import systems
+    from quantom_amd.engine import Engine
+    P = problem
+    other = systems.build_system(0x51A7 + 91, 1, 2, 3.4, 4, 320, compound_size=16)
+    ofc, ofs, opot = oracle_frames(other, K_FRAMES)
+    ofc = (ofc * 1.01).astype(np.float32)
+    ofs = (ofs * 0.99).astype(np.float32)
+    b = Engine([tiny_system, other], PARAMS)
+    lb, gb = b.adjoint(K_FRAMES, [P["alpha"], 1e-9], [P["beta"], 2e-12], np.stack([P["et"], opot], 1),
+                       np.concatenate([P["fct"], ofc], 1), np.concatenate([P["fst"], ofs], 1))
+    for r, (s, et, fct, fst, a, be) in enumerate([(tiny_system, P["et"], P["fct"], P["fst"], P["alpha"], P["beta"]),
+                                                 (other, opot, ofc, ofs, 1e-9, 2e-12)]):
+        e = Engine([s], PARAMS)
+        l1, g1 = e.adjoint(K_FRAMES, [a], [be], et[:, None], fct, fst)
+        np.testing.assert_allclose(lb[r], l1[0], rtol=1e-9)
+        np.testing.assert_allclose(gb[r], g1[0], rtol=1e-5, atol=1e-6 * np.abs(g1).max())
+        e.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_torch_node_through_engine_and_descent(tiny_system, problem):
+    from quantom_amd.adjoint import trajectory_loss
+    from quantom_amd.engine import Engine
+    P = problem
+    e = Engine([tiny_system], PARAMS)
+    p = torch.tensor(P["v0"], dtype=torch.float64, requires_grad=True)
+    kw = dict(energy_weight=[P["alpha"]], force_weight=[P["beta"]], energy_target=P["et"][:, None],
+              chain_force_target=P["fct"], solvent_force_target=P["fst"])
+    loss = trajectory_loss(e, p, K_FRAMES, **kw)
+    loss.backward()
+    _, g = e.adjoint(K_FRAMES, **{k: v for k, v in kw.items()})
+    np.testing.assert_array_equal(p.grad.numpy(), g[0])
+    # a small step down the sigma/epsilon gradient lowers the loss
+    mask = np.zeros(N_PARAMETERS)
+    mask[:64] = 1.0
+    d = p.grad.numpy() * mask
+    step = 1e-3 * np.abs(P["v0"][:64]).max() / np.abs(d).max()
+    with torch.no_grad():
+        l2 = trajectory_loss(e, torch.tensor(P["v0"] - step * d), K_FRAMES, **kw)
+    assert float(l2) < float(loss)
+    e.close()
