@@ -147,9 +147,125 @@ def accuracy_report(args):
     return out
 
 
+def adjoint_bench(args):
+    """64 replicas of 2,000 particles: gradients of an energy/force MSE over 200 frames (checkpoints every
+    20), targets from the same engine at seeded perturbed parameters (sigma +-3%, epsilon +-10%).
+    Reports gradients per second, the cost of the same gradients by central differences of forward runs,
+    and a check of replica 0 against FP64 central differences of the oracle."""
+    import numpy as np
+
+    from quantom_amd import systems
+    from quantom_amd.abi import THERMOSTAT_NONE, RunParams
+    from quantom_amd.engine import Engine
+
+    K, R, cpi = args.adj_frames, args.adj_replicas, 20
+    syss = systems.adjoint_replicas(R)
+    params = RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE)
+    eng = Engine(syss, params)
+    base = np.stack([eng.get_parameters(r) for r in range(R)])
+    n_types = 16
+    lj = [t for t in range(1, n_types + 1)] + [32 + t for t in range(1, n_types + 1)]   # sigma, epsilon of the 16 chain types
+    theta = lj + [64, 65, 66, 67]
+    rng = np.random.default_rng(0xAD1)
+    pert = base.copy()
+    for r in range(R):
+        pert[r, 1:n_types + 1] *= 1.0 + 0.03 * rng.uniform(-1, 1, n_types)
+        pert[r, 33:33 + n_types] *= 1.0 + 0.10 * rng.uniform(-1, 1, n_types)
+        eng.set_parameters(pert[r], r)
+    t0 = time.perf_counter()
+    fc, fs, pot = eng.record_frames(K)
+    log(f"targets: {K} frames of {R} replicas in {time.perf_counter() - t0:.1f}s")
+    for r in range(R):
+        eng.set_parameters(base[r], r)
+    n_r = np.array([s.n_chain + s.n_solvents for s in syss])
+    f2 = np.zeros(R)
+    cb = np.cumsum([0] + [s.n_chain for s in syss])
+    sb = np.cumsum([0] + [s.n_solvents for s in syss])
+    for r in range(R):
+        f2[r] = (fc[:, cb[r]:cb[r + 1]] ** 2).sum() + (fs[:, sb[r]:sb[r + 1]] ** 2).sum()
+    beta = 1.0 / f2                                         # mean-squared force error, normalised
+    alpha = 1.0 / (K * np.maximum(pot.var(axis=0), 1.0))    # mean-squared energy error, normalised
+    eng.adjoint(K, alpha, beta, pot, fc, fs, checkpoint_interval=cpi)          # warm
+    t0 = time.perf_counter()
+    loss, grad = eng.adjoint(K, alpha, beta, pot, fc, fs, checkpoint_interval=cpi)
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    eng.adjoint(K, alpha, beta, pot, fc, fs, checkpoint_interval=cpi, gradient=False)
+    t_fwd = time.perf_counter() - t0
+    n_fd = 2 * len(theta)
+    out = {"metric": "adjoint grads/sec", "value": R / t_adj, "unit": "gradients/s (36 parameters each)",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f32 (f64 reductions)", "data": "synthetic",
+           "config": {"workload": f"adjoint: {R} replicas x (512 chain + 1488 solvent), 4.2 nm, 16 types",
+                      "frames": K, "checkpoint_interval": cpi, "nlist_interval": 10, "parameters": len(theta)},
+           "adjoint_s_per_call": t_adj, "forward_loss_s_per_call": t_fwd,
+           "fd_equivalent_grads_per_s": R / (n_fd * t_fwd), "adjoint_over_fd": n_fd * t_fwd / t_adj,
+           "mean_loss": float(loss.mean())}
+    log(f"adjoint {t_adj:.3f}s/call ({R / t_adj:.1f} grads/s), forward {t_fwd:.3f}s; FD would take {n_fd} forward runs")
+    if args.adj_fd > 0:
+        from oracle.oracle import Oracle
+        s0 = syss[0]
+        sel = [1, 2, 3, 33, 34, 35, 0, 32, 64, 65, 66, 67][:args.adj_fd]
+
+        def oloss(v):
+            s = s0.copy()
+            s.forcefield = s.forcefield.copy()
+            s.forcefield[:, 1] = v[:32]
+            s.forcefield[:, 2] = v[32:64]
+            for a in (s.bonds, s.bridge_bonds):
+                a["kb"] = v[64]
+                a["b0"] = v[65]
+            for a in (s.angles, s.bridge_angles):
+                a["k_theta"] = v[66]
+                a["theta_0"] = v[67]
+            o = Oracle(s, params, precision=64, fast=True)
+            L = 0.0
+            for k in range(K):
+                c, so, _ = o.compute_forces()
+                e = o.energies()["potential"]
+                L += alpha[0] * (e - pot[k, 0]) ** 2 + beta[0] * (((c - fc[k, :cb[1]]) ** 2).sum() +
+                                                                   ((so - fs[k, :sb[1]]) ** 2).sum())
+                if k + 1 < K:
+                    o.step(1)
+            return L
+
+        t0 = time.perf_counter()
+        rows = []
+        for i in sel:
+            vp, vm = base[0].copy(), base[0].copy()
+            h = 1e-5 * abs(base[0, i])
+            vp[i] = np.float32(base[0, i] + h)
+            vm[i] = np.float32(base[0, i] - h)
+            fd = (oloss(vp) - oloss(vm)) / (float(vp[i]) - float(vm[i]))
+            rows.append({"param": int(i), "adjoint": float(grad[0, i]), "fd_fp64": fd,
+                         "rel_err": abs(grad[0, i] - fd) / max(abs(fd), 1e-300)})
+            log(f"  param {i}: adjoint {grad[0, i]:.6e} fd {fd:.6e}")
+        out["fd_check_replica0"] = rows
+        out["fd_check_max_rel_err"] = max(r["rel_err"] for r in rows)
+        out["fd_check_seconds"] = time.perf_counter() - t0
+    return out
+
+
+def copy_probe(args):
+    """A plain device copy of known size (the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE readings)."""
+    import torch
+    n = args.copy_mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.copy_reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"mode": "copy", "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4,
+            "launches": args.copy_reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", default="bench", choices=["bench", "accuracy"])
+    ap.add_argument("--mode", default="bench", choices=["bench", "accuracy", "copy", "adjoint"])
+    ap.add_argument("--adj-frames", type=int, default=200)
+    ap.add_argument("--adj-replicas", type=int, default=64)
+    ap.add_argument("--adj-fd", type=int, default=12, help="parameters of replica 0 checked by FP64 differences")
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--drift-steps", type=int, default=1000)
     ap.add_argument("--sequential", action="store_true",
                     help="replicas workload: also time the replicas as sequential single-system engines")
@@ -173,6 +289,12 @@ def main():
 
     if args.mode == "accuracy":
         print(json.dumps(accuracy_report(args)), flush=True)
+        return
+    if args.mode == "adjoint":
+        print(json.dumps(adjoint_bench(args)), flush=True)
+        return
+    if args.mode == "copy":
+        print(json.dumps(copy_probe(args)), flush=True)
         return
 
     t_gen = time.perf_counter()

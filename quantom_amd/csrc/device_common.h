@@ -116,43 +116,54 @@ LIMA_DEV void integrate(float& px, float& py, float& pz, float& qx, float& qy, f
 }
 
 // ---- LJ pair (calcLJForce, Engine.cu:475-522), fast form -------------------
-// sig2 = sigma*sigma, e24 = 24*epsilon, both precomputed per type pair with the
-// reference's float rounding.  1/r^2 uses the hardware reciprocal (1 ulp)
-// instead of two IEEE divisions.
-// The scalar 24 eps s/r^2 (1 - 2s) is evaluated as (s*inv) * (e24 + m2 * s) with
-// m2 = -2*e24 stored beside e24: one instruction less per pair (16 VALU + 1 rcp).
-LIMA_DEV float lj_scalar(float r2, float sig2, float e24, float m2, float& inv_out) {
+// The reference's scalar 24 eps s^3 (1 - 2 s^3) / r^2, s = sigma^2 / r^2
+// (Engine.cu:488-490), is evaluated as inv^4 (A + B inv^3) with inv = 1/r^2 and
+// the per-type-pair constants A = 24 eps sigma^6, B = -48 eps sigma^12 (lj_ab,
+// from sigma^2 and 24 eps rounded as the reference rounds them).  1/r^2 is the
+// hardware reciprocal (1 ulp) instead of two IEEE divisions.  Five multiplies or
+// FMAs after the reciprocal, one fewer than through s.
+LIMA_DEV float2 lj_ab(float sig2, float e24) {
+    const float s6 = sig2 * sig2 * sig2;
+    return make_float2(e24 * s6, -2.0f * e24 * s6 * s6);
+}
+
+LIMA_DEV float lj_scalar(float r2, float A, float B, float& inv_out) {
     const float inv = __builtin_amdgcn_rcpf(r2);
-    const float s = sig2 * inv;
-    const float s3 = s * s * s;
+    const float i2 = inv * inv;
+    const float i3 = i2 * inv;
+    const float i4 = i2 * i2;
     inv_out = inv;
-    return (s3 * inv) * __builtin_fmaf(m2, s3, e24);
+    return i4 * __builtin_fmaf(B, i3, A);
 }
 
-LIMA_DEV void lj_acc(float px, float py, float pz, float qx, float qy, float qz, float sig2, float e24, float m2,
-                     float& fx, float& fy, float& fz) {
+// FP64 pair energy 4 eps (s^6 - s^3) = -(inv^3 / 6) (A + B inv^3 / 2) of one directed pair
+LIMA_DEV double lj_energy(float r2, float A, float B) {
+    const double i = 1.0 / (double)r2, i3 = i * i * i;
+    return -(i3 / 6.0) * ((double)A + 0.5 * (double)B * i3);
+}
+
+LIMA_DEV void lj_acc(float px, float py, float pz, float qx, float qy, float qz, float A, float B, float& fx,
+                     float& fy, float& fz) {
     float dx = qx - px, dy = qy - py, dz = qz - pz;
     float r2 = dx * dx + dy * dy + dz * dz;
     float inv;
-    float fs = lj_scalar(r2, sig2, e24, m2, inv);
+    float fs = lj_scalar(r2, A, B, inv);
     fx += dx * fs;
     fy += dy * fs;
     fz += dz * fs;
 }
 
-// same pair with FP64 energy 4*eps*(s^2 - s) and virial d.F accumulation
-LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, float qz, float sig2, float e24,
-                            float m2, float& fx, float& fy, float& fz, double& e, double& w) {
+// same pair with FP64 energy and virial d.F accumulation
+LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, float qz, float A, float B, float& fx,
+                            float& fy, float& fz, double& e, double& w) {
     float dx = qx - px, dy = qy - py, dz = qz - pz;
     float r2 = dx * dx + dy * dy + dz * dz;
     float inv;
-    float fs = lj_scalar(r2, sig2, e24, m2, inv);
+    float fs = lj_scalar(r2, A, B, inv);
     fx += dx * fs;
     fy += dy * fs;
     fz += dz * fs;
-    double sd = (double)sig2 / (double)r2;
-    double s6 = sd * sd * sd;
-    e += (double)e24 / 6.0 * (s6 * s6 - s6);
+    e += lj_energy(r2, A, B);
     w += (double)r2 * (double)fs;
 }
 

@@ -273,7 +273,7 @@ struct lima_engine {
     }
 
     // ---- launches ----
-    size_t chunk_lds_bytes() const { return (size_t)chunk_cap * (sizeof(float4) + sizeof(float2)); }
+    size_t chunk_lds_bytes() const { return (size_t)chunk_cap * (sizeof(float4) + sizeof(float)); }
 
     template <bool FIXED, bool ENERGY> void launch_solvent_chain(const StepArgs& a, hipStream_t st) {
         const int nw = (int)scwork_h.size();

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 // solventForceKernel's compound loop (Engine.cu:1099-1117): every solvent
 // against every chain particle of its replica, no cutoff.  Work item =
 // (solvent tile, compound chunk).  The chunk's particles are staged in LDS as
-// (x, y, z, sigma^2) + 24*eps; every lane holds SPT solvents.  The solvent's own
+// (x, y, z, A) + B (the pair constants of lj_ab); every lane holds SPT solvents.  The solvent's own
 // position is re-imaged against each compound's particle 0 in compound order,
 // exactly as the reference mutates it (Engine.cu:1111), by replaying the
 // compounds before the chunk.  Per-compound partial sums match
@@ -182,23 +182,26 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     // dynamic LDS sized to the largest chunk of this engine: [cap] float4 + [cap] float2
     extern __shared__ float4 dyn_lds[];
     float4* tile = dyn_lds;
-    float2* tile_e = reinterpret_cast<float2*>(dyn_lds + a.chunk_cap);
+    float* tile_b = reinterpret_cast<float*>(dyn_lds + a.chunk_cap);
     __shared__ int tile_off[kChunkCompounds + 1];
-    __shared__ float4 tab[kMaxTypes];
+    __shared__ float2 tab[kMaxTypes];
     const SCWork wk = a.scwork[blockIdx.x];
     const ReplicaInfo rp = a.reps[wk.replica];
     const float L = rp.L, half = rp.half;
     const int tid = threadIdx.x;
-    if (tid < kMaxTypes) tab[tid] = a.tab_sc[wk.replica * kMaxTypes + tid];
+    if (tid < kMaxTypes) {
+        const float4 v = a.tab_sc[wk.replica * kMaxTypes + tid];
+        tab[tid] = lj_ab(v.x, v.y);
+    }
     const int c0 = wk.comp_begin, ncomp = wk.comp_end - wk.comp_begin;
     const int base = a.comp_off[c0], total = a.comp_off[wk.comp_end] - base;
     for (int k = tid; k <= ncomp; k += blockDim.x) tile_off[k] = a.comp_off[c0 + k] - base;
     __syncthreads();
     for (int k = tid; k < total; k += blockDim.x) {
         float4 p = a.cpos[base + k];
-        float4 tp = tab[ftype(p)];
-        tile[k] = make_float4(p.x, p.y, p.z, tp.x);
-        tile_e[k] = make_float2(tp.y, tp.z);
+        const float2 ab = tab[ftype(p)];
+        tile[k] = make_float4(p.x, p.y, p.z, ab.x);
+        tile_b[k] = ab.y;
     }
     float sx[SPT], sy[SPT], sz[SPT];
     float fx[SPT], fy[SPT], fz[SPT];
@@ -231,11 +234,11 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 #pragma unroll 4
         for (int j = j0; j < j1; j++) {
             const float4 q = tile[j];
-            const float2 e = tile_e[j];
+            const float b = tile_b[j];
 #pragma unroll
             for (int k = 0; k < SPT; k++) {
-                if (ENERGY) lj_acc_energy(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e.x, e.y, lx[k], ly[k], lz[k], en[k], vir[k]);
-                else lj_acc(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, e.x, e.y, lx[k], ly[k], lz[k]);
+                if (ENERGY) lj_acc_energy(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, b, lx[k], ly[k], lz[k], en[k], vir[k]);
+                else lj_acc(sx[k], sy[k], sz[k], q.x, q.y, q.z, q.w, b, lx[k], ly[k], lz[k]);
             }
         }
 #pragma unroll
@@ -281,7 +284,7 @@ LIMA_DEV int find_compound(const int* cw_off, int n, int w) {
 template <bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
     __shared__ float4 src[kSolventChunk];
-    __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (sigma^2, 24 eps): 8-byte entries keep the per-lane reads conflict-light
+    __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (A, B) of lj_ab: 8-byte entries keep the per-lane reads conflict-light
     __shared__ int cached_rep;
     const int p = threadIdx.x;
     if (p == 0) cached_rep = -1;
@@ -306,7 +309,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             if (cached_rep != r) {
                 for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
                     const float4 v = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + t];
-                    tab[t] = make_float2(v.x, v.y);
+                    tab[t] = lj_ab(v.x, v.y);
                 }
             }
             const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
@@ -341,13 +344,12 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                         float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
                         float r2 = dx * dx + dy * dy + dz * dz;
                         float inv;
-                        float fs = lj_scalar(r2, pr.x, pr.y, -2.0f * pr.y, inv);
+                        float fs = lj_scalar(r2, pr.x, pr.y, inv);
                         const bool ex = (mw >> (j & 31)) & 1u;
                         fs = ex ? 0.0f : fs;
                         lx += dx * fs; ly += dy * fs; lz += dz * fs;
                         if (ENERGY && !ex) {
-                            double sd = (double)pr.x / (double)r2, s6 = sd * sd * sd;
-                            en += (double)pr.y / 6.0 * (s6 * s6 - s6);
+                            en += lj_energy(r2, pr.x, pr.y);
                             vir += (double)r2 * (double)fs;
                         }
                     }
@@ -367,12 +369,13 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                 src[jj] = q;
             }
             __syncthreads();
-            const float4 pr = a.tab_cs[r * kMaxTypes + tp];
+            const float4 pv = a.tab_cs[r * kMaxTypes + tp];
+            const float2 pr = lj_ab(pv.x, pv.y);
             if (active) {
                 for (int j = 0; j < nsrc; j++) {
                     const float4 q = src[j];
-                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, pr.z, lx, ly, lz, en, vir);
-                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, pr.z, lx, ly, lz);
+                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz, en, vir);
+                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz);
                 }
             }
             if (ENERGY) {
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
     pos[p] = me;
     for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
         const float4 v = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + t];
-        tab[t] = make_float2(v.x, v.y);
+        tab[t] = lj_ab(v.x, v.y);
     }
     __syncthreads();
     double eb = 0, ea = 0, ed = 0;
@@ -478,13 +481,12 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
                 float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 float inv;
-                float fs = lj_scalar(r2, pr.x, pr.y, -2.0f * pr.y, inv);
+                float fs = lj_scalar(r2, pr.x, pr.y, inv);
                 const bool ex = (j == p) || ((mw >> (j & 31)) & 1u);
                 fs = ex ? 0.0f : fs;
                 lx += dx * fs; ly += dy * fs; lz += dz * fs;
                 if (ENERGY && !ex) {
-                    double sd = (double)pr.x / (double)r2, s6 = sd * sd * sd;
-                    e_intra += (double)pr.y / 6.0 * (s6 * s6 - s6);
+                    e_intra += lj_energy(r2, pr.x, pr.y);
                     vir += (double)r2 * (double)fs;
                 }
             }
@@ -570,7 +572,8 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
         float4 q = a.cpos[a.comp_off[c]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
     }
-    const float4 pss = a.tab_ss[r];
+    const float4 pv = a.tab_ss[r];
+    const float2 pss = lj_ab(pv.x, pv.y);
     float lx = 0.f, ly = 0.f, lz = 0.f;
     double en = 0, vir = 0;
     const int cnt = active ? a.ss_cnt[si] : 0;
@@ -583,15 +586,15 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
-            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, pss.z, lx, ly, lz, en, vir);
-            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, pss.z, lx, ly, lz);
+            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
+            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
         }
     }
     for (; k < cnt; k++) {
         const float4 q = a.spos[ids[k * ld]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, pss.z, lx, ly, lz, en, vir);
-        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, pss.z, lx, ly, lz);
+        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
+        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
     }
     if (active) {
         a.ss_part[s] = make_float4(lx, ly, lz, 0.f);

@@ -260,12 +260,17 @@ def test_record_frames_and_zero_loss_at_own_targets(tiny_system):
     np.testing.assert_array_equal(fc[0], cf0)
     np.testing.assert_array_equal(fs[0], sf0)
     ofc, ofs, opot = oracle_frames(tiny_system, K_FRAMES)
-    np.testing.assert_allclose(pot[:, 0], opot, rtol=1e-5)
+    # an FP32 trajectory against the FP64 one: the potential is a sum with heavy cancellation,
+    # so the bar is relative to the size of the series (measured: 3e-5 of it after 14 frames)
+    assert np.abs(pot[:, 0] - opot).max() <= 1e-4 * np.abs(opot).max()
     den = np.sqrt((ofc ** 2).sum() + (ofs ** 2).sum())
     assert np.sqrt(((fc - ofc) ** 2).sum() + ((fs - ofs) ** 2).sum()) / den < 1e-4
     loss, grad = e.adjoint(K_FRAMES, [1.0], [1.0], pot, fc, fs)
     assert loss[0] <= 1e-12 * (pot ** 2).sum()
-    assert np.all(np.abs(grad) <= 1e-6 * (1.0 + np.abs(grad).max()))
+    # at its own targets the gradient vanishes up to the rounding of the recomputed forces; the scale is
+    # the gradient for a 1% misfit of the same targets
+    _, gref = e.adjoint(K_FRAMES, [1.0], [1.0], pot * 1.01, fc * 1.01, fs * 1.01)
+    assert np.abs(grad).max() <= 1e-5 * np.abs(gref).max()
     e.close()
 
 
@@ -304,12 +309,13 @@ def test_torch_node_through_engine_and_descent(tiny_system, problem):
     loss.backward()
     _, g = e.adjoint(K_FRAMES, **{k: v for k, v in kw.items()})
     np.testing.assert_array_equal(p.grad.numpy(), g[0])
-    # a small step down the sigma/epsilon gradient lowers the loss
-    mask = np.zeros(N_PARAMETERS)
-    mask[:64] = 1.0
-    d = p.grad.numpy() * mask
-    step = 1e-3 * np.abs(P["v0"][:64]).max() / np.abs(d).max()
+    # a small step down the sigma/epsilon gradient, scaled per parameter (d_i = -eta v_i^2 g_i, at most
+    # 1e-3 of any parameter), lowers the loss
+    v0 = P["v0"]
+    d = np.zeros(N_PARAMETERS)
+    d[:64] = -(v0[:64] ** 2) * p.grad.numpy()[:64]
+    eta = 1e-3 / np.max(np.abs(d[:64]) / np.maximum(np.abs(v0[:64]), 1e-30))
     with torch.no_grad():
-        l2 = trajectory_loss(e, torch.tensor(P["v0"] - step * d), K_FRAMES, **kw)
+        l2 = trajectory_loss(e, torch.tensor(v0 + eta * d), K_FRAMES, **kw)
     assert float(l2) < float(loss)
     e.close()
