@@ -49,23 +49,55 @@ def read_traffic(kernel_tag):
     return None
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(systems_, params, budget_s):
+    """The oracle (FP32, OpenMP on all host threads) on a bounded sample of the same system: three
+    windows of steps without a rebuild (median of the window means) plus one rebuild timed alone and
+    amortised over the rebuild interval."""
+    import statistics
+
     from oracle.oracle import Oracle, num_threads
     sysm = systems_[0]
+    iv = max(1, params.nlist_interval)
     o = Oracle(sysm, params, precision=32, fast=True)
-    o.step(1)                      # warm
-    steps, t = 0, 0.0
-    block = max(1, params.nlist_interval)
-    while steps < 4 * block and (t < budget_s or steps == 0):
-        t0 = time.perf_counter()
-        o.step(block)
-        t += time.perf_counter() - t0
-        steps += block
-    ms = 1000.0 * t / steps
+    t0 = time.perf_counter()
+    o.rebuild_neighbors()
+    t_reb = time.perf_counter() - t0
+    o.step(1)                                        # warm
+    windows, steps = [], 0
+    t_start = time.perf_counter()
+    while len(windows) < 3:
+        # steps that do not end on a rebuild (the oracle rebuilds after step k when k % iv == 0)
+        n = 0
+        t = 0.0
+        while n < max(1, iv - 1):
+            if (o.step_count + 1) % iv == 0:
+                o.step(1)                            # rebuild step: not part of the windows
+                continue
+            t1 = time.perf_counter()
+            o.step(1)
+            t += time.perf_counter() - t1
+            n += 1
+            if time.perf_counter() - t_start > budget_s * (len(windows) + 1) / 3.0:
+                break
+        windows.append(t / n)
+        steps += n
+    ms = 1000.0 * (statistics.median(windows) + t_reb / iv)
     return {"value": 172.8 * (sysm.dt / 2e-6) / ms, "unit": "ns/day", "ms_per_step": ms,
-            "cores": num_threads(True), "kind": "port",
-            "sample": f"{steps} steps ({steps // block} neighbour rebuilds) of the same system, "
-                      f"oracle/liblima_oracle_fast.so (g++ -O3 -fopenmp, FMA allowed)"}
+            "cores": num_threads(True), "nproc": os.cpu_count(), "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"3 windows ({steps} steps without a rebuild, median of window means) plus one "
+                      f"neighbour rebuild ({1000 * t_reb:.1f} ms) amortised over {iv} steps; "
+                      f"oracle/liblima_oracle_fast.so (g++ -O3 -fopenmp, FMA allowed), same system"}
 
 
 def sequential_baseline(syss, params, steps):
@@ -158,7 +190,7 @@ def adjoint_bench(args):
     from quantom_amd.abi import THERMOSTAT_NONE, RunParams
     from quantom_amd.engine import Engine
 
-    K, R, cpi = args.adj_frames, args.adj_replicas, 20
+    K, R, cpi = args.adj_frames, args.adj_replicas, args.adj_checkpoint
     syss = systems.adjoint_replicas(R)
     params = RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE)
     eng = Engine(syss, params)
@@ -263,18 +295,21 @@ def main():
     ap.add_argument("--mode", default="bench", choices=["bench", "accuracy", "copy", "adjoint"])
     ap.add_argument("--adj-frames", type=int, default=200)
     ap.add_argument("--adj-replicas", type=int, default=64)
+    ap.add_argument("--adj-checkpoint", type=int, default=20, help="frames per checkpointed segment")
     ap.add_argument("--adj-fd", type=int, default=12, help="parameters of replica 0 checked by FP64 differences")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--drift-steps", type=int, default=1000)
     ap.add_argument("--sequential", action="store_true",
                     help="replicas workload: also time the replicas as sequential single-system engines")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of timing the candidates (about 20 ms)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="production")
-    ap.add_argument("--accumulation", default="fp32", choices=["fp32", "fixed64"])
+    ap.add_argument("--accumulation", default=None, choices=["fp32", "fixed64"],
+                    help="default: fixed64 for the large workload, fp32 otherwise")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
@@ -302,9 +337,13 @@ def main():
     log(f"generated {args.workload} in {time.perf_counter() - t_gen:.1f}s")
     n_chain = sum(s.n_chain for s in syss)
     n_solv = sum(s.n_solvents for s in syss)
-    params = RunParams(nlist_interval=10, accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
+    if args.accumulation is None:
+        args.accumulation = "fixed64" if args.workload == "large" else "fp32"
+    interval = 20 if args.workload == "large" else 10
+    params = RunParams(nlist_interval=interval,
+                       accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
     eng = Engine(syss, params)
-    if args.autotune:
+    if not args.no_autotune:
         t_at = time.perf_counter()
         eng.autotune(10)
         log(f"autotune {time.perf_counter() - t_at:.1f}s")

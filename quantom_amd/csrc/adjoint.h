@@ -255,33 +255,38 @@ struct AdjArgs {
 // weight as seen from P (w_P when P receives the force, -w_receiver when P is
 // the source), gU = weight of the site's energy.  Adds the position gradient of
 // u.f + gU U to acc and the derivatives with respect to (sigma^2, 24 eps).
-LIMA_DEV void site(float3 d, float3 u, float gU, float sig2, float e24, float3& acc, float& dsig2, float& de24) {
+// Sums over a particle's thousands of pairs cancel heavily, so the gathers
+// accumulate in FP64 (the per-pair terms stay FP32).
+struct D3 {
+    double x, y, z;
+};
+LIMA_DEV void site(float3 d, float3 u, float gU, float sig2, float e24, D3& acc, double& dsig2, double& de24) {
     const PairDeriv p = pair_deriv(d.x * d.x + d.y * d.y + d.z * d.z, sig2, e24);
     const float3 h = pair_h(p, d, u);
     const float ge = gU * p.phi;
-    acc.x -= h.x + ge * d.x;
-    acc.y -= h.y + ge * d.y;
-    acc.z -= h.z + ge * d.z;
+    acc.x -= (double)(h.x + ge * d.x);
+    acc.y -= (double)(h.y + ge * d.y);
+    acc.z -= (double)(h.z + ge * d.z);
     float de, dsg;
     pair_param(p, u.x * d.x + u.y * d.y + u.z * d.z, gU, sig2, e24, de, dsg);
-    dsig2 += dsg;
-    de24 += de;
+    dsig2 += (double)dsg;
+    de24 += (double)de;
 }
 
-LIMA_DEV void bins_zero(float* bins) {
-    for (int i = threadIdx.x; i < kAdjTypeBins; i += blockDim.x) bins[i] = 0.f;
+LIMA_DEV void bins_zero(double* bins) {
+    for (int i = threadIdx.x; i < kAdjTypeBins; i += blockDim.x) bins[i] = 0.0;
     __syncthreads();
 }
 // type t's credits: sigma, arithmetic 24eps, geometric 24eps (already times 24eps)
-LIMA_DEV void bins_credit(float* bins, int t, float s, float ea, float eg) {
-    if (s != 0.f) atomicAdd(&bins[t], s);
-    if (ea != 0.f) atomicAdd(&bins[kMaxTypes + t], ea);
-    if (eg != 0.f) atomicAdd(&bins[2 * kMaxTypes + t], eg);
+LIMA_DEV void bins_credit(double* bins, int t, double s, double ea, double eg) {
+    if (s != 0.0) atomicAdd(&bins[t], s);
+    if (ea != 0.0) atomicAdd(&bins[kMaxTypes + t], ea);
+    if (eg != 0.0) atomicAdd(&bins[2 * kMaxTypes + t], eg);
 }
-LIMA_DEV void bins_flush(const float* bins, double* g) {
+LIMA_DEV void bins_flush(const double* bins, double* g) {
     __syncthreads();
     for (int i = threadIdx.x; i < kAdjTypeBins; i += blockDim.x)
-        if (bins[i] != 0.f) atomicAdd(&g[i], (double)bins[i]);
+        if (bins[i] != 0.0) atomicAdd(&g[i], bins[i]);
 }
 LIMA_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 LIMA_DEV float3 neg3(float3 v) { return make_float3(-v.x, -v.y, -v.z); }
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
     const StepArgs& a = A.s;
     __shared__ float4 cp[kAdjCompChunk * kCompoundBlock];
     __shared__ float4 cw[kAdjCompChunk * kCompoundBlock];
-    __shared__ float bins[kAdjTypeBins];
+    __shared__ double bins[kAdjTypeBins];
     bins_zero(bins);
     const int4 tile = a.stiles[blockIdx.x];
     const int r = tile.x;
@@ -311,8 +316,8 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
     const float3 w = xyz(A.ws[si]);
     const float g = A.gE[r];
     const int sl = si - rp.solv_begin;
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    float bs = 0.f, bea = 0.f, beg = 0.f;
+    D3 acc{0.0, 0.0, 0.0};
+    double bs = 0.0, bea = 0.0, beg = 0.0;
     const int c0 = rp.comp_begin + blockIdx.y * kAdjCompChunk;
     const int c1 = min(c0 + kAdjCompChunk, rp.comp_end);
     if (c0 < c1) {
@@ -332,14 +337,14 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
                 float4 q = cp[j];
                 const int tj = ftype(q);
                 const float4 pa = a.tab_sc[r * kMaxTypes + tj];
-                float ds = 0.f, de = 0.f;
+                double ds = 0.0, de = 0.0;
                 site(make_float3(q.x - ix, q.y - iy, q.z - iz), w, 0.5f * g, pa.x, pa.y, acc, ds, de);
                 bs += ds * pa.w;
                 bea += de;
                 if (listed) {
                     hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
                     const float4 pg = a.tab_cs[r * kMaxTypes + tj];
-                    ds = 0.f; de = 0.f;
+                    ds = 0.0; de = 0.0;
                     site(make_float3(q.x - ix, q.y - iy, q.z - iz), neg3(xyz(cw[j])), 0.5f * g, pg.x, pg.y, acc, ds, de);
                     bs += ds * pg.w;
                     beg += de * pg.y;
@@ -351,11 +356,11 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
         const float4 pss = a.tab_ss[r];
         const int cnt = active ? a.ss_cnt[si] : 0;
         for (int k = 0; k < cnt; k++) {
-            const int j = a.ss_ids[(size_t)k * a.n_solv + si];
+            const int j = a.ss_ids[(size_t)si * a.ss_ld + k];
             float4 q = a.spos[j];
             hyperpos(me.x, me.y, me.z, q.x, q.y, q.z, L, half);
             const float3 wj = xyz(A.ws[j]);
-            float ds = 0.f, de = 0.f;
+            double ds = 0.0, de = 0.0;
             site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), make_float3(w.x - wj.x, w.y - wj.y, w.z - wj.z), g,
                  pss.x, pss.y, acc, ds, de);
             bs += ds * pss.w;
@@ -363,7 +368,7 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
         }
     }
     if (active) {
-        A.gxs[(size_t)blockIdx.y * a.n_solv + s] = make_float4(acc.x, acc.y, acc.z, 0.f);
+        A.gxs[(size_t)blockIdx.y * a.n_solv + s] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, 0.f);
         bins_credit(bins, 0, bs, bea, beg);
     }
     bins_flush(bins, A.gpar + (size_t)r * kAdjStride);
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(128) void adj_chain_solvent_kernel(AdjArgs A) {
     const StepArgs& a = A.s;
     __shared__ float4 sp[kSolventChunk];
     __shared__ float4 sw[kSolventChunk];
-    __shared__ float bins[kAdjTypeBins];
+    __shared__ double bins[kAdjTypeBins];
     bins_zero(bins);
     const int c = blockIdx.x, p = threadIdx.x;
     const int r = a.comp_rep[c];
@@ -393,8 +398,8 @@ __global__ __launch_bounds__(128) void adj_chain_solvent_kernel(AdjArgs A) {
     const int tp = ftype(me);
     const float4 pa = a.tab_sc[r * kMaxTypes + tp];
     const float g = A.gE[r];
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    float bs = 0.f, bea = 0.f;
+    D3 acc{0.0, 0.0, 0.0};
+    double bs = 0.0, bea = 0.0;
     const int s0 = rp.solv_begin + blockIdx.y * kAdjSolvSlice;
     const int s1 = min(s0 + kAdjSolvSlice, rp.solv_end);
     for (int t0 = s0; t0 < s1; t0 += kSolventChunk) {
@@ -413,8 +418,8 @@ __global__ __launch_bounds__(128) void adj_chain_solvent_kernel(AdjArgs A) {
         }
     }
     if (active) {
-        A.gxc[(size_t)(1 + blockIdx.y) * A.n_chain + gi] = make_float4(acc.x, acc.y, acc.z, 0.f);
-        bins_credit(bins, tp, bs * pa.w, bea, 0.f);
+        A.gxc[(size_t)(1 + blockIdx.y) * A.n_chain + gi] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, 0.f);
+        bins_credit(bins, tp, bs * pa.w, bea, 0.0);
     }
     bins_flush(bins, A.gpar + (size_t)r * kAdjStride);
 }
@@ -433,7 +438,7 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
     __shared__ float4 pos[kCompoundBlock];
     __shared__ float4 wv[kCompoundBlock];
     __shared__ uint32_t nbmask[kCompoundBlock * 4];   // neighbour's exclusions of this compound's particles
-    __shared__ float bins[kAdjTypeBins];
+    __shared__ double bins[kAdjTypeBins];
     __shared__ float3 gb[kMaxBondsPerCompound * 2];
     __shared__ float3 ga[kMaxAnglesPerCompound * 3];
     __shared__ float3 gd[kMaxDihedralsPerCompound * 4];
@@ -454,13 +459,13 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
     const int tp = ftype(me);
     pos[p] = me;
     wv[p] = make_float4(w.x, w.y, w.z, 0.f);
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    float bs = 0.f, bea = 0.f, beg = 0.f;
+    D3 acc{0.0, 0.0, 0.0};
+    double bs = 0.0, bea = 0.0, beg = 0.0;
     // (1) listed solvents: "particle <- solvent", geometric eps (Engine.cu:964-974)
     {
         const float4 pg = a.tab_cs[r * kMaxTypes + tp];
         const int ncs = a.enable_solvents ? a.cs_cnt[c] : 0;
-        float ds = 0.f, de = 0.f;
+        double ds = 0.0, de = 0.0;
         for (int j0 = 0; j0 < ncs; j0 += kSolventChunk) {
             const int ns = min(kSolventChunk, ncs - j0);
             __syncthreads();
@@ -513,7 +518,7 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
             const float3 wj = xyz(sw[j]);
             const float3 u = make_float3((in_p ? w.x : 0.f) - (in_j ? wj.x : 0.f), (in_p ? w.y : 0.f) - (in_j ? wj.y : 0.f),
                                          (in_p ? w.z : 0.f) - (in_j ? wj.z : 0.f));
-            float ds = 0.f, de = 0.f;
+            double ds = 0.0, de = 0.0;
             site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), u, 0.5f * g * (float)((int)in_p + (int)in_j), pr.x,
                  pr.y, acc, ds, de);
             bs += ds * pr.w;
@@ -536,7 +541,7 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
             const float3 wj = xyz(wv[j]);
             const float3 u = make_float3((in_p ? w.x : 0.f) - (in_j ? wj.x : 0.f), (in_p ? w.y : 0.f) - (in_j ? wj.y : 0.f),
                                          (in_p ? w.z : 0.f) - (in_j ? wj.z : 0.f));
-            float ds = 0.f, de = 0.f;
+            double ds = 0.0, de = 0.0;
             site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), u, 0.5f * g * (float)((int)in_p + (int)in_j), pr.x,
                  pr.y, acc, ds, de);
             bs += ds * pr.w;
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
             acc.y += x.y;
             acc.z += x.z;
         }
-        A.gxc[gi] = make_float4(acc.x, acc.y, acc.z, 0.f);
+        A.gxc[gi] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, 0.f);
         bins_credit(bins, tp, bs, bea, beg);
     }
     double* gr = A.gpar + (size_t)r * kAdjStride;
@@ -717,21 +722,24 @@ __global__ __launch_bounds__(256) void adj_residual_kernel(const float4* f, cons
 }
 
 // w_k = (dt^2/m) a_{k+1} + r_k
-__global__ __launch_bounds__(256) void adj_weight_kernel(const float4* resid, const float4* a_next, const float* cfac,
+__global__ __launch_bounds__(256) void adj_weight_kernel(const float4* resid, const double4* a_next, const float* cfac,
                                                          float4* w, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 r = resid[i], an = a_next[i];
-    const float c = cfac[i];
-    w[i] = make_float4(an.x * c + r.x, an.y * c + r.y, an.z * c + r.z, 0.f);
+    const float4 r = resid[i];
+    const double4 an = a_next[i];
+    const double c = cfac[i];
+    w[i] = make_float4((float)(an.x * c + r.x), (float)(an.y * c + r.y), (float)(an.z * c + r.z), 0.f);
 }
 
-// a_k = G_x + 2 a_{k+1} - a_{k+2}, G_x summed over its parts in slot order
+// a_k = G_x + 2 a_{k+1} - a_{k+2} in FP64 (the recurrence runs over the whole
+// trajectory), G_x summed over its parts in slot order
 __global__ __launch_bounds__(256) void adj_update_kernel(const float4* gxc, int nc_slots, const float4* gxs, int ns_slots,
-                                                         const float4* a1, const float4* a2, float4* out, int nch, int ns) {
+                                                         const double4* a1, const double4* a2, double4* out, int nch,
+                                                         int ns) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nch + ns) return;
-    float gx = 0.f, gy = 0.f, gz = 0.f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
     if (i < nch) {
         for (int k = 0; k < nc_slots; k++) {
             const float4 v = gxc[(size_t)k * nch + i];
@@ -743,8 +751,8 @@ __global__ __launch_bounds__(256) void adj_update_kernel(const float4* gxc, int 
             gx += v.x; gy += v.y; gz += v.z;
         }
     }
-    const float4 x = a1[i], y = a2[i];
-    out[i] = make_float4(gx + 2.f * x.x - y.x, gy + 2.f * x.y - y.y, gz + 2.f * x.z - y.z, 0.f);
+    const double4 x = a1[i], y = a2[i];
+    out[i] = make_double4(gx + 2.0 * x.x - y.x, gy + 2.0 * x.y - y.y, gz + 2.0 * x.z - y.z, 0.0);
 }
 
 }  // namespace lima

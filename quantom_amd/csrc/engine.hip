@@ -83,7 +83,7 @@ struct AdjBufs {
     DBuf<int> ck_cc;          // [segments][NC + NC*cap_cc]: compound lists before the segment's rebuild
     DBuf<int> lists;          // [slots][list words]: slot 0 the entry lists, then a segment's periods
     DBuf<float4> X, Rs, T;    // [segment frames][N]: positions, 2 beta (F - F*), targets F*
-    DBuf<float4> adj;         // [3][N] a_k, a_{k+1}, a_{k+2} in rotation
+    DBuf<double4> adj;        // [3][N] a_k, a_{k+1}, a_{k+2} in rotation (FP64)
     DBuf<float4> w;           // [N]
     DBuf<float4> gxc, gxs;    // position-gradient parts
     DBuf<float> cfac, gE;     // [N] dt^2/m, [K][R] energy weights
@@ -227,7 +227,7 @@ struct lima_engine {
         a.cap_ss = params.max_solvent_neighbors;
         a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p;
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
-        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
+        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
@@ -260,7 +260,7 @@ struct lima_engine {
         a.cc_cnt = cc_cnt.p; a.cc_ids = cc_ids.p;
         a.seg_cnt = seg_cnt.p; a.max_segs = max_segs;
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
-        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p;
+        a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.n_cells = n_cells;
         a.cell_cnt = cell_cnt.p; a.cell_start = cell_start.p; a.cell_cursor = cell_cursor.p;
         a.cell_ids = cell_ids.p; a.solv_cell = solv_cell.p;
@@ -271,6 +271,9 @@ struct lima_engine {
         a.enable_solvents = params.enable_solvents;
         return a;
     }
+
+    // solvent list row stride: capacity rounded up to 4 ints (16-byte aligned rows)
+    int ss_ld() const { return (params.max_solvent_neighbors + 3) & ~3; }
 
     // ---- launches ----
     size_t chunk_lds_bytes() const { return (size_t)chunk_cap * (sizeof(float4) + sizeof(float)); }
@@ -770,7 +773,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->seg_cnt.alloc(NCx * E->max_segs); E->seg_cnt.zero();
     E->cs_ids.alloc(NCx * P.max_compound_solvents);
     E->ss_cnt.alloc(NSx); E->ss_cnt.zero();
-    E->ss_ids.alloc(NSx * P.max_solvent_neighbors);
+    E->ss_ids.alloc(NSx * E->ss_ld());
     E->cw_off.alloc(NCx + 1); E->cw_off.zero();
     E->cw_total.alloc(1); E->cw_total.zero();
     E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
@@ -812,7 +815,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
 size_t list_words(const lima_engine* E) {
     const lima_run_params& P = E->params;
     return (size_t)E->NC * (2 + P.max_compound_neighbors + P.max_compound_solvents) +
-           (size_t)E->NS * (1 + P.max_solvent_neighbors);
+           (size_t)E->NS * (1 + E->ss_ld());
 }
 
 void d2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
@@ -829,7 +832,7 @@ void save_lists(lima_engine* E, int* dst) {
     d2d(dst, E->cs_cnt.p, nc * 4, st); dst += nc;
     d2d(dst, E->cs_ids.p, nc * P.max_compound_solvents * 4, st); dst += nc * P.max_compound_solvents;
     d2d(dst, E->ss_cnt.p, ns * 4, st); dst += ns;
-    d2d(dst, E->ss_ids.p, ns * P.max_solvent_neighbors * 4, st);
+    d2d(dst, E->ss_ids.p, ns * E->ss_ld() * 4, st);
 }
 void load_lists(lima_engine* E, const int* src) {
     const lima_run_params& P = E->params;
@@ -840,7 +843,7 @@ void load_lists(lima_engine* E, const int* src) {
     d2d(E->cs_cnt.p, src, nc * 4, st); src += nc;
     d2d(E->cs_ids.p, src, nc * P.max_compound_solvents * 4, st); src += nc * P.max_compound_solvents;
     d2d(E->ss_cnt.p, src, ns * 4, st); src += ns;
-    d2d(E->ss_ids.p, src, ns * P.max_solvent_neighbors * 4, st);
+    d2d(E->ss_ids.p, src, ns * E->ss_ld() * 4, st);
     if (E->NC > 0) nl_work_kernel<<<1, 1024, 0, st>>>(E->nlist_args(E->cur));   // listed work list of these lists
 }
 size_t cc_words(const lima_engine* E) { return (size_t)E->NC * (1 + E->params.max_compound_neighbors); }
@@ -981,9 +984,9 @@ void adj_step(lima_engine* E, bool energy) {
 void adj_frame(lima_engine* E, int j, const float4* X, const float4* rs, const float* gE, bool solv) {
     AdjBufs& B = E->adjb;
     const int nch = E->NCH, ns = E->NS, N = nch + ns;
-    float4* a_out = B.adj.p + (size_t)(j % 3) * N;
-    const float4* a1 = B.adj.p + (size_t)((j + 1) % 3) * N;
-    const float4* a2 = B.adj.p + (size_t)((j + 2) % 3) * N;
+    double4* a_out = B.adj.p + (size_t)(j % 3) * N;
+    const double4* a1 = B.adj.p + (size_t)((j + 1) % 3) * N;
+    const double4* a2 = B.adj.p + (size_t)((j + 2) % 3) * N;
     adj_weight_kernel<<<(N + 255) / 256, 256, 0, E->stream>>>(rs, a1, B.cfac.p, B.w.p, N);
     AdjArgs A{};
     A.s = E->step_args(E->cur);
@@ -1116,7 +1119,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
     // ---- backward pass ----
     if (st == LIMA_OK && grad_out) {
         HIP_TRY(hipMemcpy(B.gE.p, gE.data(), gE.size() * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemsetAsync(B.adj.p, 0, (size_t)3 * N * 16, E->stream));
+        HIP_TRY(hipMemsetAsync(B.adj.p, 0, (size_t)3 * N * sizeof(double4), E->stream));
         HIP_TRY(hipMemsetAsync(B.gpar.p, 0, (size_t)R * kAdjStride * 8, E->stream));
         for (int s = nseg - 1; s >= 0; s--) {
             const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
@@ -1585,10 +1588,11 @@ int lima_get_neighbors(lima_engine* E, int32_t* cci, int32_t* csi, int32_t* ssi)
         if (cci && E->NC) HIP_TRY(hipMemcpy(cci, E->cc_ids.p, (size_t)E->NC * P.max_compound_neighbors * sizeof(int), hipMemcpyDeviceToHost));
         if (csi && E->NC) HIP_TRY(hipMemcpy(csi, E->cs_ids.p, (size_t)E->NC * P.max_compound_solvents * sizeof(int), hipMemcpyDeviceToHost));
         if (ssi && E->NS) {
-            std::vector<int> t((size_t)E->NS * P.max_solvent_neighbors);
+            const size_t ld = E->ss_ld();
+            std::vector<int> t((size_t)E->NS * ld);
             HIP_TRY(hipMemcpy(t.data(), E->ss_ids.p, t.size() * sizeof(int), hipMemcpyDeviceToHost));
             for (int s = 0; s < E->NS; s++)
-                for (int k = 0; k < P.max_solvent_neighbors; k++) ssi[(size_t)s * P.max_solvent_neighbors + k] = t[(size_t)k * E->NS + s];
+                for (int k = 0; k < P.max_solvent_neighbors; k++) ssi[(size_t)s * P.max_solvent_neighbors + k] = t[(size_t)s * ld + k];
         }
         return LIMA_OK;
     });
@@ -1638,9 +1642,10 @@ int lima_set_neighbors(lima_engine* E, const int32_t* ccc, const int32_t* cci, c
                     if (j < ri.solv_begin || j >= ri.solv_end || j == s) throw LimaError(LIMA_E_ARG, "solvent neighbour id outside the solvent's system");
                 }
             }
-            std::vector<int> t((size_t)E->NS * P.max_solvent_neighbors, 0);
+            const size_t ld = E->ss_ld();
+            std::vector<int> t((size_t)E->NS * ld, 0);
             for (int s = 0; s < E->NS; s++)
-                for (int k = 0; k < ssc[s]; k++) t[(size_t)k * E->NS + s] = ssi[(size_t)s * P.max_solvent_neighbors + k];
+                for (int k = 0; k < ssc[s]; k++) t[(size_t)s * ld + k] = ssi[(size_t)s * P.max_solvent_neighbors + k];
             HIP_TRY(hipMemcpy(E->ss_cnt.p, ssc, E->NS * sizeof(int), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(E->ss_ids.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
         }

@@ -70,7 +70,8 @@ struct StepArgs {
     int cap_cc, cap_cs, cap_ss;
     const int* cc_cnt; const int* cc_ids;
     const int* cs_cnt; const int* cs_ids;
-    const int* ss_cnt; const int* ss_ids;   // transposed: ss_ids[k*n_solv + s]
+    const int* ss_cnt; const int* ss_ids;   // rows: ss_ids[s*ss_ld + k], ss_ld = cap_ss rounded up to 4
+    int ss_ld;
     float4* ss_part;              // solvent-solvent local sums
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
@@ -334,7 +335,26 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             }
             __syncthreads();
             if (p == 0) cached_rep = r;
-            if (active) {
+            // exclusions across compounds exist only between bridged compounds: a wave
+            // with none against this neighbour takes the loop without the mask test
+            const bool wave_clean = __ballot((m0 | m1 | m2 | m3) != 0u) == 0ull;
+            if (active && wave_clean) {
+                const float2* trow = tab + tp;
+#pragma unroll 4
+                for (int j = 0; j < nn; j++) {
+                    const float4 q = src[j];
+                    const float2 pr = trow[ftype(q) * kMaxTypes];
+                    const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    float inv;
+                    const float fs = lj_scalar(r2, pr.x, pr.y, inv);
+                    lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                    if (ENERGY) {
+                        en += lj_energy(r2, pr.x, pr.y);
+                        vir += (double)r2 * (double)fs;
+                    }
+                }
+            } else if (active) {
                 for (int g = 0; g * 32 < nn; g++) {
                     const uint32_t mw = g == 0 ? m0 : g == 1 ? m1 : g == 2 ? m2 : m3;
                     const int jend = min(nn, g * 32 + 32);
@@ -577,12 +597,11 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
     float lx = 0.f, ly = 0.f, lz = 0.f;
     double en = 0, vir = 0;
     const int cnt = active ? a.ss_cnt[si] : 0;
-    const int* ids = a.ss_ids + si;
-    const size_t ld = (size_t)a.n_solv;
+    const int* ids = a.ss_ids + (size_t)si * a.ss_ld;   // 16-byte aligned rows
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
-        const int i0 = ids[(k + 0) * ld], i1 = ids[(k + 1) * ld], i2 = ids[(k + 2) * ld], i3 = ids[(k + 3) * ld];
-        const float4 q[4] = {a.spos[i0], a.spos[i1], a.spos[i2], a.spos[i3]};
+        const int4 i4 = *reinterpret_cast<const int4*>(ids + k);
+        const float4 q[4] = {a.spos[i4.x], a.spos[i4.y], a.spos[i4.z], a.spos[i4.w]};
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
@@ -591,7 +610,7 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
         }
     }
     for (; k < cnt; k++) {
-        const float4 q = a.spos[ids[k * ld]];
+        const float4 q = a.spos[ids[k]];
         hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
         if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
         else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
@@ -725,7 +744,8 @@ struct NlistArgs {
     int cap_cc, cap_cs, cap_ss;
     int* cc_cnt; int* cc_ids;
     int* cs_cnt; int* cs_ids;
-    int* ss_cnt; int* ss_ids;
+    int* ss_cnt; int* ss_ids;   // rows of ss_ld
+    int ss_ld;
     int n_cells;
     int* cell_cnt; int* cell_start; int* cell_cursor; int* cell_ids; int* solv_cell;
     const int* cell_rep;          // replica of each cell
@@ -995,7 +1015,7 @@ __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
                 const bool member = e < total && ss_member(s, me, j, cand_pos[e < total ? e : 0], L, half, bound);
                 const unsigned long long ball = __ballot(member);
                 const int pos = count + __popcll(ball & ((1ull << lane) - 1ull));
-                if (member && pos < a.cap_ss) a.ss_ids[(size_t)pos * a.n_solv + s] = j;
+                if (member && pos < a.cap_ss) a.ss_ids[(size_t)s * a.ss_ld + pos] = j;
                 count += __popcll(ball);
             }
             if (lane == 0) {
@@ -1015,7 +1035,7 @@ __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
                 for (int e = a.cell_start[nb[k]]; e < e1; e++) {
                     const int j = a.cell_ids[e];
                     if (ss_member(s, me, j, a.spos[j], L, half, bound)) {
-                        if (count < a.cap_ss) a.ss_ids[(size_t)count * a.n_solv + s] = j;
+                        if (count < a.cap_ss) a.ss_ids[(size_t)s * a.ss_ld + count] = j;
                         count++;
                     }
                 }

@@ -308,7 +308,8 @@ def test_torch_node_through_engine_and_descent(tiny_system, problem):
     loss = trajectory_loss(e, p, K_FRAMES, **kw)
     loss.backward()
     _, g = e.adjoint(K_FRAMES, **{k: v for k, v in kw.items()})
-    np.testing.assert_array_equal(p.grad.numpy(), g[0])
+    # FP64 atomics make two runs agree to rounding, not bit for bit
+    np.testing.assert_allclose(p.grad.numpy(), g[0], rtol=1e-12, atol=1e-300)
     # a small step down the sigma/epsilon gradient, scaled per parameter (d_i = -eta v_i^2 g_i, at most
     # 1e-3 of any parameter), lowers the loss
     v0 = P["v0"]
