@@ -342,6 +342,8 @@ def main():
     interval = 20 if args.workload == "large" else 10
     params = RunParams(nlist_interval=interval,
                        accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
+    if args.workload == "large":
+        params.max_compound_solvents = 32768     # compounds confined to 0.38 L span up to ~17k list entries
     eng = Engine(syss, params)
     if not args.no_autotune:
         t_at = time.perf_counter()

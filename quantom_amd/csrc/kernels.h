@@ -967,6 +967,7 @@ LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, 
 __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
     __shared__ float4 cand_pos[kCellCandidates];
     __shared__ int cand_id[kCellCandidates];
+    __shared__ int cell_b[27], cell_n[27], cell_off[28];
     const int cell = blockIdx.x;
     const int h0 = a.cell_start[cell], h1 = a.cell_start[cell + 1];
     if (h0 == h1) return;
@@ -976,38 +977,52 @@ __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
     const int local = cell - rp.cell_begin;
     const int cx = local / (n * n), cy = (local / n) % n, cz = local % n;
     const int span = n >= 3 ? 3 : n;
-    int nb[27];
-    int nnb = 0;
-    for (int ox = 0; ox < span; ox++) {
+    const int nnb = span * span * span;
+    // neighbour cell k = (ox, oy, oz) in a fixed order; grids narrower than 3 cells take every cell once
+    auto nb_cell = [&](int k) {
+        const int ox = k / (span * span), oy = (k / span) % span, oz = k % span;
         const int x = (n >= 3) ? (cx - 1 + ox + n) % n : ox;
-        for (int oy = 0; oy < span; oy++) {
-            const int y = (n >= 3) ? (cy - 1 + oy + n) % n : oy;
-            for (int oz = 0; oz < span; oz++) {
-                const int z = (n >= 3) ? (cz - 1 + oz + n) % n : oz;
-                nb[nnb++] = rp.cell_begin + (x * n + y) * n + z;
-            }
-        }
-    }
-    int total = 0;
-    for (int k = 0; k < nnb; k++) {
-        const int b = a.cell_start[nb[k]], e = a.cell_start[nb[k] + 1];
-        for (int j = threadIdx.x; j < e - b; j += blockDim.x) {
-            if (total + j < kCellCandidates) {
-                const int id = a.cell_ids[b + j];
-                cand_id[total + j] = id;
-                cand_pos[total + j] = a.spos[id];
-            }
-        }
-        total += e - b;
+        const int y = (n >= 3) ? (cy - 1 + oy + n) % n : oy;
+        const int z = (n >= 3) ? (cz - 1 + oz + n) % n : oz;
+        return rp.cell_begin + (x * n + y) * n + z;
+    };
+    const int k_home = (n >= 3) ? 13 : (cx * span + cy) * span + cz;
+    // all cell ranges at once, then an exclusive scan of their sizes
+    if (threadIdx.x < nnb) {
+        const int c = nb_cell(threadIdx.x);
+        const int b = a.cell_start[c];
+        cell_b[threadIdx.x] = b;
+        cell_n[threadIdx.x] = a.cell_start[c + 1] - b;
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < nnb; k++) {
+            cell_off[k] = acc;
+            acc += cell_n[k];
+        }
+        cell_off[nnb] = acc;
+    }
+    __syncthreads();
+    const int total = cell_off[nnb];
     if (total <= kCellCandidates) {
-        // each wavefront takes home solvents in turn; its lanes test 64 candidates
-        // at a time and append the members in candidate order (ballot compaction)
+        // stage the candidates in the fixed cell order (independent loads, no per-cell round trips)
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            int k = 0;
+            while (cell_off[k + 1] <= e) k++;
+            const int id = a.cell_ids[cell_b[k] + (e - cell_off[k])];
+            cand_id[e] = id;
+            cand_pos[e] = a.spos[id];
+        }
+        __syncthreads();
+        // each wavefront takes home solvents (staged with the home cell) in turn; its lanes test
+        // 64 candidates at a time and append the members in candidate order (ballot compaction)
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (int h = h0 + wid; h < h1; h += nw) {
-            const int s = a.cell_ids[h];
-            const float4 me = a.spos[s];
+        const int home0 = cell_off[k_home], nhome = h1 - h0;
+        for (int h = wid; h < nhome; h += nw) {
+            const int s = cand_id[home0 + h];
+            const float4 me = cand_pos[home0 + h];
+            int* row = a.ss_ids + (size_t)s * a.ss_ld;
             int count = 0;
             for (int e0 = 0; e0 < total; e0 += 64) {
                 const int e = e0 + lane;
@@ -1015,7 +1030,7 @@ __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
                 const bool member = e < total && ss_member(s, me, j, cand_pos[e < total ? e : 0], L, half, bound);
                 const unsigned long long ball = __ballot(member);
                 const int pos = count + __popcll(ball & ((1ull << lane) - 1ull));
-                if (member && pos < a.cap_ss) a.ss_ids[(size_t)s * a.ss_ld + pos] = j;
+                if (member && pos < a.cap_ss) row[pos] = j;
                 count += __popcll(ball);
             }
             if (lane == 0) {
@@ -1031,8 +1046,8 @@ __global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
         int count = 0;
         {   // more candidates than LDS holds: same order, read from global memory
             for (int k = 0; k < nnb; k++) {
-                const int e1 = a.cell_start[nb[k] + 1];
-                for (int e = a.cell_start[nb[k]]; e < e1; e++) {
+                const int b = cell_b[k], e1 = b + cell_n[k];
+                for (int e = b; e < e1; e++) {
                     const int j = a.cell_ids[e];
                     if (ss_member(s, me, j, a.spos[j], L, half, bound)) {
                         if (count < a.cap_ss) a.ss_ids[(size_t)s * a.ss_ld + count] = j;
