@@ -72,6 +72,9 @@ struct StepArgs {
     const int* cs_cnt; const int* cs_ids;
     const int* ss_cnt; const int* ss_ids;   // rows: ss_ids[s*ss_ld + k], ss_ld = cap_ss rounded up to 4
     int ss_ld;
+    float4* simg;                 // [kmax+1][n_solv] the solvent re-imaged against the compounds before chunk k; [kmax]: all
+    const int* chunk_start;       // first compound of each chunk, per replica from ReplicaInfo::chunk_base
+    int kmax;
     float4* ss_part;              // solvent-solvent local sums
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
@@ -169,14 +172,41 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The solvent's own position is re-imaged against each compound's particle 0 in
+// compound order before its LJ sums (Engine.cu:1111).  One thread per solvent
+// runs that sequence once per step and stores the image before each chunk of
+// the all-pairs kernel (simg[k]) and after the last compound (simg[kmax]), so
+// the all-pairs items and the listed kernel start from it with the same bits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n_solv) return;
+    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
+    const float4 me = a.spos[s];
+    float x = me.x, y = me.y, z = me.z;
+    int k = 0;
+    int next = rp.chunk_count > 0 ? a.chunk_start[rp.chunk_base] : -1;
+    for (int c = rp.comp_begin; c < rp.comp_end; c++) {
+        if (c == next) {
+            a.simg[(size_t)k * a.n_solv + s] = make_float4(x, y, z, 0.f);
+            k++;
+            next = k < rp.chunk_count ? a.chunk_start[rp.chunk_base + k] : -1;
+        }
+        const float4 q = a.cpos[a.comp_off[c]];
+        hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
+    }
+    a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // solventForceKernel's compound loop (Engine.cu:1099-1117): every solvent
 // against every chain particle of its replica, no cutoff.  Work item =
 // (solvent tile, compound chunk).  The chunk's particles are staged in LDS as
 // (x, y, z, A) + B (the pair constants of lj_ab); every lane holds SPT solvents.  The solvent's own
 // position is re-imaged against each compound's particle 0 in compound order,
-// exactly as the reference mutates it (Engine.cu:1111), by replaying the
-// compounds before the chunk.  Per-compound partial sums match
-// computeLJForces' local sum (Engine.cu:640-651).
+// exactly as the reference mutates it (Engine.cu:1111): the item starts from the
+// image solvent_replay_kernel stored before its chunk.  Per-compound partial sums
+// match computeLJForces' local sum (Engine.cu:640-651).
 // ---------------------------------------------------------------------------
 template <int SPT, bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
@@ -211,17 +241,12 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
         int s = wk.solv_begin + tid + k * blockDim.x;
-        float4 p = a.spos[s < wk.solv_end ? s : wk.solv_begin];
+        // the solvent after the reference's sequential re-imaging over the preceding compounds
+        float4 p = a.simg[(size_t)wk.chunk * a.n_solv + (s < wk.solv_end ? s : wk.solv_begin)];
         sx[k] = p.x; sy[k] = p.y; sz[k] = p.z;
         fx[k] = fy[k] = fz[k] = 0.f;
         ix[k] = iy[k] = iz[k] = 0;
         en[k] = vir[k] = 0.0;
-    }
-    // replay the reference's sequential re-imaging over the preceding compounds
-    for (int c = rp.comp_begin; c < c0; c++) {
-        float4 q = a.cpos[a.comp_off[c]];
-#pragma unroll
-        for (int k = 0; k < SPT; k++) hyperpos(q.x, q.y, q.z, sx[k], sy[k], sz[k], L, half);
     }
     __syncthreads();
     for (int ci = 0; ci < ncomp; ci++) {
@@ -586,12 +611,8 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
     const int s = tile.y + threadIdx.x;
     const bool active = s < tile.z;
     const int si = active ? s : tile.y;
-    const float4 me = a.spos[si];
-    float sx = me.x, sy = me.y, sz = me.z;
-    for (int c = rp.comp_begin; c < rp.comp_end; c++) {                    // Engine.cu:1111
-        float4 q = a.cpos[a.comp_off[c]];
-        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-    }
+    const float4 im = a.simg[(size_t)a.kmax * a.n_solv + si];           // after every compound, Engine.cu:1111
+    float sx = im.x, sy = im.y, sz = im.z;
     const float4 pv = a.tab_ss[r];
     const float2 pss = lj_ab(pv.x, pv.y);
     float lx = 0.f, ly = 0.f, lz = 0.f;
@@ -964,7 +985,11 @@ LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, 
     return (j != s) && dist2_reimaged(ax, ay, az, bx, by, bz, L, half) < bound;
 }
 
-__global__ __launch_bounds__(256) void nl_solvent_solvent_kernel(NlistArgs a) {
+// 1024 threads: the 80 KB candidate stage leaves one block per CU, so the block
+// itself must carry the waves that hide LDS and ballot latency (16 per CU)
+constexpr int kSolventListThreads = 1024;
+
+__global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel(NlistArgs a) {
     __shared__ float4 cand_pos[kCellCandidates];
     __shared__ int cand_id[kCellCandidates];
     __shared__ int cell_b[27], cell_n[27], cell_off[28];
