@@ -178,24 +178,36 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 // the all-pairs kernel (simg[k]) and after the last compound (simg[kmax]), so
 // the all-pairs items and the listed kernel start from it with the same bits.
 // ---------------------------------------------------------------------------
+constexpr int kReplayStage = 1024;   // particle-0 positions staged in LDS per round
+
 __global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n_solv) return;
-    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
-    const float4 me = a.spos[s];
+    // one solvent tile (a single replica) per block; its compounds' particle 0
+    // positions are staged in LDS, so the sequence runs from broadcast reads
+    __shared__ float4 p0[kReplayStage];
+    const int4 tile = a.stiles[blockIdx.x];
+    const ReplicaInfo rp = a.reps[tile.x];
+    const int s = tile.y + threadIdx.x;
+    const bool active = s < tile.z;
+    const float4 me = a.spos[active ? s : tile.y];
     float x = me.x, y = me.y, z = me.z;
     int k = 0;
     int next = rp.chunk_count > 0 ? a.chunk_start[rp.chunk_base] : -1;
-    for (int c = rp.comp_begin; c < rp.comp_end; c++) {
-        if (c == next) {
-            a.simg[(size_t)k * a.n_solv + s] = make_float4(x, y, z, 0.f);
-            k++;
-            next = k < rp.chunk_count ? a.chunk_start[rp.chunk_base + k] : -1;
+    for (int c0 = rp.comp_begin; c0 < rp.comp_end; c0 += kReplayStage) {
+        const int nc = min(kReplayStage, rp.comp_end - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) p0[i] = a.cpos[a.comp_off[c0 + i]];
+        __syncthreads();
+        for (int i = 0; i < nc; i++) {
+            if (c0 + i == next) {
+                if (active) a.simg[(size_t)k * a.n_solv + s] = make_float4(x, y, z, 0.f);
+                k++;
+                next = k < rp.chunk_count ? a.chunk_start[rp.chunk_base + k] : -1;
+            }
+            const float4 q = p0[i];
+            hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
         }
-        const float4 q = a.cpos[a.comp_off[c]];
-        hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
     }
-    a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
+    if (active) a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
 }
 
 // ---------------------------------------------------------------------------
