@@ -37,7 +37,6 @@ struct ReplicaInfo {
     int solv_begin, solv_end;     // global solvent ids
     int stile_begin, stile_end;   // solvent integrate tiles (KE partials)
     int chunk_count;              // solvent<-chain compound chunks of this replica
-    int chunk_base;               // first entry of this replica's chunk starts (StepArgs::chunk_start)
     int cell_begin;               // first cell of this replica's solvent grid
     int ncell;                    // cells per dimension
     float cell_inv;               // ncell / L

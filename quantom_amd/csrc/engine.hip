@@ -151,7 +151,7 @@ struct lima_engine {
     DBuf<uint32_t> err;
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
     DBuf<float4> ss_part, ss_img, simg;
-    DBuf<int> chunk_start;
+    int n_img = 0;                // stored re-imaging images per solvent before the final one
     DBuf<float> comp_cs_bound;
     AdjBufs adjb;
 
@@ -230,7 +230,7 @@ struct lima_engine {
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
-        a.simg = simg.p; a.chunk_start = chunk_start.p; a.kmax = kmax;
+        a.simg = simg.p; a.kmax = n_img;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
@@ -296,7 +296,6 @@ struct lima_engine {
     void build_scwork() {
         const int tile_sz = 256 * cfg.spt;
         scwork_h.clear();
-        std::vector<int> starts;
         kmax = 0;
         chunk_cap = 1;
         for (int r = 0; r < R; r++) {
@@ -317,8 +316,6 @@ struct lima_engine {
             }
             if (ri.solv_end == ri.solv_begin) chunks.clear();
             ri.chunk_count = (int)chunks.size();
-            ri.chunk_base = (int)starts.size();
-            for (const auto& ch : chunks) starts.push_back(ch.first);
             kmax = std::max(kmax, ri.chunk_count);
             for (int t0 = ri.solv_begin; t0 < ri.solv_end; t0 += tile_sz)
                 for (int k = 0; k < (int)chunks.size(); k++)
@@ -338,8 +335,10 @@ struct lima_engine {
             if (!partS_fx.p) partS_fx.alloc(1);
         }
         scwork.upload(scwork_h);
-        chunk_start.upload(starts);
-        const size_t nimg = (size_t)(kmax + 1) * std::max(NS, 1);
+        n_img = 0;
+        for (const ReplicaInfo& ri : reps_h)
+            n_img = std::max(n_img, (ri.comp_end - ri.comp_begin + kReplayEvery - 1) / kReplayEvery);
+        const size_t nimg = (size_t)(n_img + 1) * std::max(NS, 1);
         if (simg.n < nimg) simg.alloc(nimg);
     }
 
@@ -350,7 +349,7 @@ struct lima_engine {
         HIP_TRY(hipEventRecord(ev_fork, stream));
         HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
         // main: the solvents' sequential re-imaging, read by both solvent kernels
-        if (solv) solvent_replay_kernel<<<n_stiles, 256, 0, stream>>>(a);
+        if (solv) solvent_replay_kernel<<<4 * n_stiles, 64, 0, stream>>>(a);
         HIP_TRY(hipEventRecord(ev_replay, stream));
         HIP_TRY(hipStreamWaitEvent(s2, ev_replay, 0));
         // s2: bridge carry-in forces, then the listed solvent-solvent LJ
@@ -1705,7 +1704,7 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
         for (int k = 0; k < 8; k++) out8[k] = 0.0;
         // the re-imaging pass first: the two solvent kernels below read its images
         if (E->params.enable_solvents && E->NS > 0)
-            out8[7] = timeit([&] { solvent_replay_kernel<<<E->n_stiles, 256, 0, E->stream>>>(a); });
+            out8[7] = timeit([&] { solvent_replay_kernel<<<4 * E->n_stiles, 64, 0, E->stream>>>(a); });
         const int nw = (int)E->scwork_h.size();
         if (E->params.enable_solvents && nw > 0) {
             out6[0] = timeit([&] {

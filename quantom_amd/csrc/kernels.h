@@ -72,9 +72,8 @@ struct StepArgs {
     const int* cs_cnt; const int* cs_ids;
     const int* ss_cnt; const int* ss_ids;   // rows: ss_ids[s*ss_ld + k], ss_ld = cap_ss rounded up to 4
     int ss_ld;
-    float4* simg;                 // [kmax+1][n_solv] the solvent re-imaged against the compounds before chunk k; [kmax]: all
-    const int* chunk_start;       // first compound of each chunk, per replica from ReplicaInfo::chunk_base
-    int kmax;
+    float4* simg;                 // [kmax+1][n_solv] the solvent re-imaged against the compounds before every 16th; [kmax]: all
+    int kmax;                     // stored images per solvent before the last (max over replicas)
     float4* ss_part;              // solvent-solvent local sums
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
@@ -174,38 +173,29 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 // ---------------------------------------------------------------------------
 // The solvent's own position is re-imaged against each compound's particle 0 in
 // compound order before its LJ sums (Engine.cu:1111).  One thread per solvent
-// runs that sequence once per step and stores the image before each chunk of
-// the all-pairs kernel (simg[k]) and after the last compound (simg[kmax]), so
-// the all-pairs items and the listed kernel start from it with the same bits.
+// runs that sequence once per step and stores the image before every 16th
+// compound (simg[j]) and after the last compound (simg[kmax]); the all-pairs
+// items continue from the nearest stored image and the listed kernel starts
+// from the last, with the same bits as the full sequence.
 // ---------------------------------------------------------------------------
-constexpr int kReplayStage = 1024;   // particle-0 positions staged in LDS per round
+constexpr int kReplayEvery = 16;     // compounds between stored images
 
-__global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
-    // one solvent tile (a single replica) per block; its compounds' particle 0
-    // positions are staged in LDS, so the sequence runs from broadcast reads
-    __shared__ float4 p0[kReplayStage];
-    const int4 tile = a.stiles[blockIdx.x];
+__global__ __launch_bounds__(64) void solvent_replay_kernel(StepArgs a) {
+    // one wavefront per block: a quarter of a 256-solvent tile (a single replica)
+    const int4 tile = a.stiles[blockIdx.x >> 2];
     const ReplicaInfo rp = a.reps[tile.x];
-    const int s = tile.y + threadIdx.x;
+    const int s = tile.y + (blockIdx.x & 3) * 64 + threadIdx.x;
+    if (tile.y + (blockIdx.x & 3) * 64 >= tile.z) return;          // wave-uniform: empty quarter
     const bool active = s < tile.z;
     const float4 me = a.spos[active ? s : tile.y];
     float x = me.x, y = me.y, z = me.z;
-    int k = 0;
-    int next = rp.chunk_count > 0 ? a.chunk_start[rp.chunk_base] : -1;
-    for (int c0 = rp.comp_begin; c0 < rp.comp_end; c0 += kReplayStage) {
-        const int nc = min(kReplayStage, rp.comp_end - c0);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) p0[i] = a.cpos[a.comp_off[c0 + i]];
-        __syncthreads();
-        for (int i = 0; i < nc; i++) {
-            if (c0 + i == next) {
-                if (active) a.simg[(size_t)k * a.n_solv + s] = make_float4(x, y, z, 0.f);
-                k++;
-                next = k < rp.chunk_count ? a.chunk_start[rp.chunk_base + k] : -1;
-            }
-            const float4 q = p0[i];
-            hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
+    for (int c = rp.comp_begin, j = 0; c < rp.comp_end; c++) {
+        if ((c - rp.comp_begin) % kReplayEvery == 0) {
+            if (active) a.simg[(size_t)j * a.n_solv + s] = make_float4(x, y, z, 0.f);
+            j++;
         }
+        const float4 q = a.cpos[a.comp_off[c]];
+        hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
     }
     if (active) a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
 }
@@ -216,9 +206,9 @@ __global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
 // (solvent tile, compound chunk).  The chunk's particles are staged in LDS as
 // (x, y, z, A) + B (the pair constants of lj_ab); every lane holds SPT solvents.  The solvent's own
 // position is re-imaged against each compound's particle 0 in compound order,
-// exactly as the reference mutates it (Engine.cu:1111): the item starts from the
-// image solvent_replay_kernel stored before its chunk.  Per-compound partial sums
-// match computeLJForces' local sum (Engine.cu:640-651).
+// exactly as the reference mutates it (Engine.cu:1111): the item continues the
+// sequence from the nearest image solvent_replay_kernel stored before its chunk.
+// Per-compound partial sums match computeLJForces' local sum (Engine.cu:640-651).
 // ---------------------------------------------------------------------------
 template <int SPT, bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
@@ -253,12 +243,18 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
         int s = wk.solv_begin + tid + k * blockDim.x;
-        // the solvent after the reference's sequential re-imaging over the preceding compounds
-        float4 p = a.simg[(size_t)wk.chunk * a.n_solv + (s < wk.solv_end ? s : wk.solv_begin)];
+        // the solvent after the reference's sequential re-imaging up to the stored image
+        // nearest before the chunk (the rest follows below)
+        float4 p = a.simg[(size_t)((c0 - rp.comp_begin) / kReplayEvery) * a.n_solv + (s < wk.solv_end ? s : wk.solv_begin)];
         sx[k] = p.x; sy[k] = p.y; sz[k] = p.z;
         fx[k] = fy[k] = fz[k] = 0.f;
         ix[k] = iy[k] = iz[k] = 0;
         en[k] = vir[k] = 0.0;
+    }
+    for (int c = rp.comp_begin + ((c0 - rp.comp_begin) / kReplayEvery) * kReplayEvery; c < c0; c++) {
+        const float4 q = a.cpos[a.comp_off[c]];
+#pragma unroll
+        for (int k = 0; k < SPT; k++) hyperpos(q.x, q.y, q.z, sx[k], sy[k], sz[k], L, half);
     }
     __syncthreads();
     for (int ci = 0; ci < ncomp; ci++) {
