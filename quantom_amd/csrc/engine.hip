@@ -348,13 +348,14 @@ struct lima_engine {
         const bool solv = params.enable_solvents && NS > 0;
         HIP_TRY(hipEventRecord(ev_fork, stream));
         HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
+        HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
         // main: the solvents' sequential re-imaging, read by both solvent kernels
-        if (solv) solvent_replay_kernel<<<4 * n_stiles, 64, 0, stream>>>(a);
+        if (solv) solvent_replay_kernel<<<n_stiles, 256, 0, stream>>>(a);
         HIP_TRY(hipEventRecord(ev_replay, stream));
-        HIP_TRY(hipStreamWaitEvent(s2, ev_replay, 0));
         // s2: bridge carry-in forces, then the listed solvent-solvent LJ
         if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_bridge, s2));
+        HIP_TRY(hipStreamWaitEvent(s2, ev_replay, 0));
         if (solv) solvent_listed_kernel<ENERGY><<<n_stiles, 256, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s1: compound listed LJ, then bonded + intra + integration of the compounds
@@ -1704,7 +1705,7 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
         for (int k = 0; k < 8; k++) out8[k] = 0.0;
         // the re-imaging pass first: the two solvent kernels below read its images
         if (E->params.enable_solvents && E->NS > 0)
-            out8[7] = timeit([&] { solvent_replay_kernel<<<4 * E->n_stiles, 64, 0, E->stream>>>(a); });
+            out8[7] = timeit([&] { solvent_replay_kernel<<<E->n_stiles, 256, 0, E->stream>>>(a); });
         const int nw = (int)E->scwork_h.size();
         if (E->params.enable_solvents && nw > 0) {
             out6[0] = timeit([&] {

@@ -179,23 +179,30 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 // from the last, with the same bits as the full sequence.
 // ---------------------------------------------------------------------------
 constexpr int kReplayEvery = 16;     // compounds between stored images
+constexpr int kReplayStage = 1024;   // particle-0 positions staged in LDS per round
 
-__global__ __launch_bounds__(64) void solvent_replay_kernel(StepArgs a) {
-    // one wavefront per block: a quarter of a 256-solvent tile (a single replica)
-    const int4 tile = a.stiles[blockIdx.x >> 2];
+__global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
+    // one solvent tile (a single replica) per block; the replica's particle-0
+    // positions are staged in LDS, so the sequence runs from broadcast reads
+    __shared__ float4 p0[kReplayStage];
+    const int4 tile = a.stiles[blockIdx.x];
     const ReplicaInfo rp = a.reps[tile.x];
-    const int s = tile.y + (blockIdx.x & 3) * 64 + threadIdx.x;
-    if (tile.y + (blockIdx.x & 3) * 64 >= tile.z) return;          // wave-uniform: empty quarter
+    const int s = tile.y + threadIdx.x;
     const bool active = s < tile.z;
     const float4 me = a.spos[active ? s : tile.y];
     float x = me.x, y = me.y, z = me.z;
-    for (int c = rp.comp_begin, j = 0; c < rp.comp_end; c++) {
-        if ((c - rp.comp_begin) % kReplayEvery == 0) {
-            if (active) a.simg[(size_t)j * a.n_solv + s] = make_float4(x, y, z, 0.f);
-            j++;
+    for (int c0 = rp.comp_begin; c0 < rp.comp_end; c0 += kReplayStage) {
+        const int nc = min(kReplayStage, rp.comp_end - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) p0[i] = a.cpos[a.comp_off[c0 + i]];
+        __syncthreads();
+        for (int i = 0; i < nc; i++) {
+            const int c = c0 + i - rp.comp_begin;
+            if (c % kReplayEvery == 0 && active)
+                a.simg[(size_t)(c / kReplayEvery) * a.n_solv + s] = make_float4(x, y, z, 0.f);
+            const float4 q = p0[i];
+            hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
         }
-        const float4 q = a.cpos[a.comp_off[c]];
-        hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
     }
     if (active) a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
 }
