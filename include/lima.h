@@ -206,8 +206,7 @@ int lima_get_timings(lima_engine* engine, double* out5);
    `reps` times alone on the engine's stream between HIP events, on the current
    state: [0] solvent<-chain all-pairs, [1] compound listed LJ, [2] compound
    bonded+intra+integrate (forces only), [3] solvent integrate (forces only),
-   [4] bridge, [5] full neighbour-list rebuild, [6] solvent listed LJ,
-   [7] the solvents' sequential re-imaging pass. */
+   [4] bridge, [5] full neighbour-list rebuild, [6] solvent listed LJ, [7] 0. */
 int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
 /* per-kernel launch shape chosen for this engine (see lima_autotune) */
 int lima_get_config(lima_engine* engine, int32_t* out8);

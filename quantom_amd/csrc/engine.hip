@@ -150,8 +150,7 @@ struct lima_engine {
     DBuf<double> ke_comp, ke_stile, temperature, energy;
     DBuf<uint32_t> err;
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
-    DBuf<float4> ss_part, ss_img, simg;
-    int n_img = 0;                // stored re-imaging images per solvent before the final one
+    DBuf<float4> ss_part, ss_img;
     DBuf<float> comp_cs_bound;
     AdjBufs adjb;
 
@@ -160,7 +159,7 @@ struct lima_engine {
     // s1:     compound listed LJ -> compound integrate
     // s2:     bridge, solvent listed LJ
     hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr, ev_replay = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
 
@@ -174,7 +173,7 @@ struct lima_engine {
 
     ~lima_engine() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c, ev_replay})
+        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c})
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t s : {stream, s1, s2})
             if (s) (void)hipStreamDestroy(s);
@@ -184,7 +183,7 @@ struct lima_engine {
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c, &ev_replay})
+        for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
 
@@ -230,7 +229,6 @@ struct lima_engine {
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
-        a.simg = simg.p; a.kmax = n_img;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
@@ -335,11 +333,6 @@ struct lima_engine {
             if (!partS_fx.p) partS_fx.alloc(1);
         }
         scwork.upload(scwork_h);
-        n_img = 0;
-        for (const ReplicaInfo& ri : reps_h)
-            n_img = std::max(n_img, (ri.comp_end - ri.comp_begin + kReplayEvery - 1) / kReplayEvery);
-        const size_t nimg = (size_t)(n_img + 1) * std::max(NS, 1);
-        if (simg.n < nimg) simg.alloc(nimg);
     }
 
     // one force pass (and integration when ADVANCE) as a fork/join over three streams
@@ -349,13 +342,9 @@ struct lima_engine {
         HIP_TRY(hipEventRecord(ev_fork, stream));
         HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
         HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
-        // main: the solvents' sequential re-imaging, read by both solvent kernels
-        if (solv) solvent_replay_kernel<<<n_stiles, 256, 0, stream>>>(a);
-        HIP_TRY(hipEventRecord(ev_replay, stream));
         // s2: bridge carry-in forces, then the listed solvent-solvent LJ
         if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_bridge, s2));
-        HIP_TRY(hipStreamWaitEvent(s2, ev_replay, 0));
         if (solv) solvent_listed_kernel<ENERGY><<<n_stiles, 256, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s1: compound listed LJ, then bonded + intra + integration of the compounds
@@ -1703,9 +1692,6 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
             return ms * 1000.0 / reps;
         };
         for (int k = 0; k < 8; k++) out8[k] = 0.0;
-        // the re-imaging pass first: the two solvent kernels below read its images
-        if (E->params.enable_solvents && E->NS > 0)
-            out8[7] = timeit([&] { solvent_replay_kernel<<<E->n_stiles, 256, 0, E->stream>>>(a); });
         const int nw = (int)E->scwork_h.size();
         if (E->params.enable_solvents && nw > 0) {
             out6[0] = timeit([&] {

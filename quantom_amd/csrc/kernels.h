@@ -72,8 +72,6 @@ struct StepArgs {
     const int* cs_cnt; const int* cs_ids;
     const int* ss_cnt; const int* ss_ids;   // rows: ss_ids[s*ss_ld + k], ss_ld = cap_ss rounded up to 4
     int ss_ld;
-    float4* simg;                 // [kmax+1][n_solv] the solvent re-imaged against the compounds before every 16th; [kmax]: all
-    int kmax;                     // stored images per solvent before the last (max over replicas)
     float4* ss_part;              // solvent-solvent local sums
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
@@ -172,39 +170,26 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
 
 // ---------------------------------------------------------------------------
 // The solvent's own position is re-imaged against each compound's particle 0 in
-// compound order before its LJ sums (Engine.cu:1111).  One thread per solvent
-// runs that sequence once per step and stores the image before every 16th
-// compound (simg[j]) and after the last compound (simg[kmax]); the all-pairs
-// items continue from the nearest stored image and the listed kernel starts
-// from the last, with the same bits as the full sequence.
+// compound order before its LJ sums (Engine.cu:1111).  replay_images continues
+// that sequence over compounds [cb, ce) for the SPT solvents of each lane, from
+// particle-0 positions staged in LDS; every thread of the block calls it.
 // ---------------------------------------------------------------------------
-constexpr int kReplayEvery = 16;     // compounds between stored images
-constexpr int kReplayStage = 1024;   // particle-0 positions staged in LDS per round
+constexpr int kReplayStage = 256;
 
-__global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
-    // one solvent tile (a single replica) per block; the replica's particle-0
-    // positions are staged in LDS, so the sequence runs from broadcast reads
-    __shared__ float4 p0[kReplayStage];
-    const int4 tile = a.stiles[blockIdx.x];
-    const ReplicaInfo rp = a.reps[tile.x];
-    const int s = tile.y + threadIdx.x;
-    const bool active = s < tile.z;
-    const float4 me = a.spos[active ? s : tile.y];
-    float x = me.x, y = me.y, z = me.z;
-    for (int c0 = rp.comp_begin; c0 < rp.comp_end; c0 += kReplayStage) {
-        const int nc = min(kReplayStage, rp.comp_end - c0);
+template <int SPT>
+LIMA_DEV void replay_images(const StepArgs& a, int cb, int ce, float4* p0s, float* sx, float* sy, float* sz, float L,
+                            float half) {
+    for (int c0 = cb; c0 < ce; c0 += kReplayStage) {
+        const int nc = min(kReplayStage, ce - c0);
         __syncthreads();
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) p0[i] = a.cpos[a.comp_off[c0 + i]];
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) p0s[i] = a.cpos[a.comp_off[c0 + i]];
         __syncthreads();
         for (int i = 0; i < nc; i++) {
-            const int c = c0 + i - rp.comp_begin;
-            if (c % kReplayEvery == 0 && active)
-                a.simg[(size_t)(c / kReplayEvery) * a.n_solv + s] = make_float4(x, y, z, 0.f);
-            const float4 q = p0[i];
-            hyperpos(q.x, q.y, q.z, x, y, z, rp.L, rp.half);
+            const float4 q = p0s[i];
+#pragma unroll
+            for (int k = 0; k < SPT; k++) hyperpos(q.x, q.y, q.z, sx[k], sy[k], sz[k], L, half);
         }
     }
-    if (active) a.simg[(size_t)a.kmax * a.n_solv + s] = make_float4(x, y, z, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -213,9 +198,9 @@ __global__ __launch_bounds__(256) void solvent_replay_kernel(StepArgs a) {
 // (solvent tile, compound chunk).  The chunk's particles are staged in LDS as
 // (x, y, z, A) + B (the pair constants of lj_ab); every lane holds SPT solvents.  The solvent's own
 // position is re-imaged against each compound's particle 0 in compound order,
-// exactly as the reference mutates it (Engine.cu:1111): the item continues the
-// sequence from the nearest image solvent_replay_kernel stored before its chunk.
-// Per-compound partial sums match computeLJForces' local sum (Engine.cu:640-651).
+// exactly as the reference mutates it (Engine.cu:1111), by replaying the
+// compounds before the chunk (replay_images).  Per-compound partial sums match
+// computeLJForces' local sum (Engine.cu:640-651).
 // ---------------------------------------------------------------------------
 template <int SPT, bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
@@ -225,6 +210,7 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     float* tile_b = reinterpret_cast<float*>(dyn_lds + a.chunk_cap);
     __shared__ int tile_off[kChunkCompounds + 1];
     __shared__ float2 tab[kMaxTypes];
+    __shared__ float4 p0s[kReplayStage];
     const SCWork wk = a.scwork[blockIdx.x];
     const ReplicaInfo rp = a.reps[wk.replica];
     const float L = rp.L, half = rp.half;
@@ -250,19 +236,14 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
         int s = wk.solv_begin + tid + k * blockDim.x;
-        // the solvent after the reference's sequential re-imaging up to the stored image
-        // nearest before the chunk (the rest follows below)
-        float4 p = a.simg[(size_t)((c0 - rp.comp_begin) / kReplayEvery) * a.n_solv + (s < wk.solv_end ? s : wk.solv_begin)];
+        float4 p = a.spos[s < wk.solv_end ? s : wk.solv_begin];
         sx[k] = p.x; sy[k] = p.y; sz[k] = p.z;
         fx[k] = fy[k] = fz[k] = 0.f;
         ix[k] = iy[k] = iz[k] = 0;
         en[k] = vir[k] = 0.0;
     }
-    for (int c = rp.comp_begin + ((c0 - rp.comp_begin) / kReplayEvery) * kReplayEvery; c < c0; c++) {
-        const float4 q = a.cpos[a.comp_off[c]];
-#pragma unroll
-        for (int k = 0; k < SPT; k++) hyperpos(q.x, q.y, q.z, sx[k], sy[k], sz[k], L, half);
-    }
+    // the reference's sequential re-imaging over the compounds before the chunk
+    replay_images<SPT>(a, rp.comp_begin, c0, p0s, sx, sy, sz, L, half);
     __syncthreads();
     for (int ci = 0; ci < ncomp; ci++) {
         const int j0 = tile_off[ci], j1 = tile_off[ci + 1];
@@ -626,8 +607,10 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
     const int s = tile.y + threadIdx.x;
     const bool active = s < tile.z;
     const int si = active ? s : tile.y;
-    const float4 im = a.simg[(size_t)a.kmax * a.n_solv + si];           // after every compound, Engine.cu:1111
-    float sx = im.x, sy = im.y, sz = im.z;
+    __shared__ float4 p0s[kReplayStage];
+    const float4 me = a.spos[si];
+    float sx = me.x, sy = me.y, sz = me.z;
+    replay_images<1>(a, rp.comp_begin, rp.comp_end, p0s, &sx, &sy, &sz, L, half);   // Engine.cu:1111
     const float4 pv = a.tab_ss[r];
     const float2 pss = lj_ab(pv.x, pv.y);
     float lx = 0.f, ly = 0.f, lz = 0.f;

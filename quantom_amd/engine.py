@@ -239,7 +239,7 @@ class Engine:
         return {"device_us": t[0], "rebuild_us": t[1], "host_us": t[2], "steps": int(t[3]), "rebuilds": int(t[4])}
 
     KERNELS = ("solvent_chain", "compound_listed", "compound_integrate", "solvent_integrate", "bridge", "rebuild",
-               "solvent_listed", "solvent_replay")
+               "solvent_listed", "reserved")
 
     def time_kernels(self, reps: int = 20) -> dict:
         """Average microseconds per launch of each step kernel, launched alone (HIP events)."""
