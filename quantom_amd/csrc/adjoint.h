@@ -235,8 +235,7 @@ LIMA_DEV void pair_param(const PairDeriv& p, float wd, float gU, float sig2, flo
 }
 
 constexpr int kAdjStride = kAdjTypeBins + 4;   // per replica: type bins, then kb, b0, k_theta, theta0
-constexpr int kAdjCompChunk = 8;               // compounds per block of the solvent-side pass
-constexpr int kAdjSolvSlice = 2048;            // solvents per block of the chain-side all-pairs pass
+constexpr int kAdjCompChunk = 8;               // most compounds per block of the solvent-side pass
 
 struct AdjArgs {
     StepArgs s;              // the frame's positions (cpos, spos), its lists and the type tables
@@ -249,6 +248,9 @@ struct AdjArgs {
     const uint32_t* cs_mask; // per compound, bitmask over its replica's solvents: listed by the compound
     const int* mask_off;     // [n_comp] first word of each compound's mask
     int n_chain;             // NCH: stride of the chain parts
+    float4* gpart;           // [work item][128]: listed-term parts of the compound side, per step work item
+    int comp_chunk;          // compounds per block of the solvent-side pass (<= kAdjCompChunk)
+    int solv_slice;          // solvents per block of the chain-side all-pairs pass (multiple of kSolventChunk)
 };
 
 // One directed LJ site seen from particle P: d = x_other - x_P, u = the force
@@ -288,11 +290,20 @@ LIMA_DEV void bins_flush(const double* bins, double* g) {
     for (int i = threadIdx.x; i < kAdjTypeBins; i += blockDim.x)
         if (bins[i] != 0.0) atomicAdd(&g[i], bins[i]);
 }
+// flush the block's bins and clear them for the next replica (each thread clears the entries it flushed)
+LIMA_DEV void bins_flush_reset(double* bins, double* g) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAdjTypeBins; i += blockDim.x) {
+        if (bins[i] != 0.0) atomicAdd(&g[i], bins[i]);
+        bins[i] = 0.0;
+    }
+    __syncthreads();
+}
 LIMA_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 LIMA_DEV float3 neg3(float3 v) { return make_float3(-v.x, -v.y, -v.z); }
 
 // ---------------------------------------------------------------------------
-// Solvent side.  Block = (solvent tile of 256, chunk of kAdjCompChunk compounds).
+// Solvent side.  Block = (solvent tile of 256, chunk of comp_chunk <= 8 compounds).
 // For every chain particle j of the chunk: the all-pairs site "solvent <- j"
 // (arithmetic eps, Engine.cu:1099-1117) and, when the solvent is in j's
 // compound list, the transpose of "j <- solvent" (geometric eps,
@@ -318,8 +329,8 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
     const int sl = si - rp.solv_begin;
     D3 acc{0.0, 0.0, 0.0};
     double bs = 0.0, bea = 0.0, beg = 0.0;
-    const int c0 = rp.comp_begin + blockIdx.y * kAdjCompChunk;
-    const int c1 = min(c0 + kAdjCompChunk, rp.comp_end);
+    const int c0 = rp.comp_begin + blockIdx.y * A.comp_chunk;
+    const int c1 = min(c0 + A.comp_chunk, rp.comp_end);
     if (c0 < c1) {
         const int pb = a.comp_off[c0], np = a.comp_off[c1] - pb;
         for (int k = threadIdx.x; k < np; k += blockDim.x) {
@@ -375,7 +386,7 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Chain side of the all-pairs site.  Block = (compound, slice of kAdjSolvSlice
+// Chain side of the all-pairs site.  Block = (compound, slice of solv_slice
 // solvents); one thread per chain particle: the transpose of "solvent <- j" with
 // the solvent imaged against the compound's particle 0 and j's stored position,
 // as the forward pass pairs them.
@@ -400,8 +411,8 @@ __global__ __launch_bounds__(128) void adj_chain_solvent_kernel(AdjArgs A) {
     const float g = A.gE[r];
     D3 acc{0.0, 0.0, 0.0};
     double bs = 0.0, bea = 0.0;
-    const int s0 = rp.solv_begin + blockIdx.y * kAdjSolvSlice;
-    const int s1 = min(s0 + kAdjSolvSlice, rp.solv_end);
+    const int s0 = rp.solv_begin + blockIdx.y * A.solv_slice;
+    const int s1 = min(s0 + A.solv_slice, rp.solv_end);
     for (int t0 = s0; t0 < s1; t0 += kSolventChunk) {
         const int ns = min(kSolventChunk, s1 - t0);
         __syncthreads();
@@ -425,19 +436,125 @@ __global__ __launch_bounds__(128) void adj_chain_solvent_kernel(AdjArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Compound side, one block per compound: its listed solvents (geometric eps),
-// neighbour compounds (arithmetic eps, both directions of each pair, LJ
-// exclusions of both particles), intra-compound pairs (geometric eps) and the
-// bonded terms (dual numbers; per-term coordinate gradients gathered per
-// particle in term order).  Bonded-constant derivatives are summed over terms.
+// Compound side of the listed terms, on the step's work list (compound, slot) as
+// compound_listed_kernel walks it: a slot below the compound's neighbour count
+// is one neighbour compound, both directions of each pair (arithmetic eps,
+// Engine.cu:653-676; the pair enters p's force unless p excludes j and j's
+// force unless j excludes p); a later slot is 256 entries of the compound's
+// solvent list, "particle <- solvent" (geometric eps, Engine.cu:964-974).  One
+// part per work item; parameter credits per replica.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
+__global__ __launch_bounds__(128) void adj_listed_kernel(AdjArgs A) {
     const StepArgs& a = A.s;
     __shared__ float4 sp[kSolventChunk];
     __shared__ float4 sw[kSolventChunk];
+    __shared__ uint32_t nbmask[kCompoundBlock * 4];   // the neighbour's exclusions of this compound's particles
+    __shared__ double bins[kAdjTypeBins];
+    bins_zero(bins);
+    const int p = threadIdx.x;
+    const int total = *a.cw_total;
+    int cur_r = -1;
+    for (int wi = blockIdx.x; wi < total; wi += gridDim.x) {
+        const int c = find_compound(a.cw_off, a.n_comp, wi);
+        const int k = wi - a.cw_off[c];
+        const int r = a.comp_rep[c];
+        if (r != cur_r) {
+            if (cur_r >= 0) bins_flush_reset(bins, A.gpar + (size_t)cur_r * kAdjStride);
+            cur_r = r;
+        }
+        const ReplicaInfo rp = a.reps[r];
+        const float L = rp.L, half = rp.half;
+        const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
+        const bool active = p < n;
+        const int gi = base + (active ? p : 0);
+        float4 me = a.cpos[gi];
+        const float4 p0 = a.cpos[base];
+        hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);
+        const float3 w = xyz(A.wc[gi]);
+        const float g = A.gE[r];
+        const int tp = ftype(me);
+        D3 acc{0.0, 0.0, 0.0};
+        double bs = 0.0, bea = 0.0, beg = 0.0;
+        const int ncc = a.cc_cnt[c];
+        __syncthreads();
+        if (k < ncc) {
+            const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
+            const int nbase = a.comp_off[nb], nn = a.comp_off[nb + 1] - nbase;
+            if (p < nn) {
+                float4 q = a.cpos[nbase + p];
+                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                sp[p] = q;
+                sw[p] = A.wc[nbase + p];
+                uint32_t m[4] = {0u, 0u, 0u, 0u};
+                for (int e = a.xoff[nbase + p]; e < a.xoff[nbase + p + 1]; e++) {
+                    const int2 x = a.xent[e];
+                    if (x.x == c) m[x.y >> 5] |= 1u << (x.y & 31);
+                }
+                for (int q4 = 0; q4 < 4; q4++) nbmask[p * 4 + q4] = m[q4];
+            }
+            __syncthreads();
+            if (active) {
+                uint32_t m[4] = {0u, 0u, 0u, 0u};
+                for (int e = a.xoff[gi]; e < a.xoff[gi + 1]; e++) {
+                    const int2 x = a.xent[e];
+                    if (x.x == nb) m[x.y >> 5] |= 1u << (x.y & 31);
+                }
+                for (int j = 0; j < nn; j++) {
+                    const bool in_p = !((m[j >> 5] >> (j & 31)) & 1u);
+                    const bool in_j = !((nbmask[j * 4 + (p >> 5)] >> (p & 31)) & 1u);
+                    if (!in_p && !in_j) continue;
+                    const float4 q = sp[j];
+                    const float4 pr = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + ftype(q) * kMaxTypes + tp];
+                    const float3 wj = xyz(sw[j]);
+                    const float3 u = make_float3((in_p ? w.x : 0.f) - (in_j ? wj.x : 0.f),
+                                                 (in_p ? w.y : 0.f) - (in_j ? wj.y : 0.f),
+                                                 (in_p ? w.z : 0.f) - (in_j ? wj.z : 0.f));
+                    double ds = 0.0, de = 0.0;
+                    site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), u, 0.5f * g * (float)((int)in_p + (int)in_j),
+                         pr.x, pr.y, acc, ds, de);
+                    bs += ds * pr.w;
+                    bea += de;
+                }
+            }
+        } else {
+            const int j0 = (k - ncc) * kSolventChunk;
+            const int ns = min(kSolventChunk, a.cs_cnt[c] - j0);
+            for (int e = p; e < ns; e += blockDim.x) {
+                float4 q = a.spos[a.cs_ids[(size_t)c * a.cap_cs + j0 + e]];
+                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                sp[e] = q;
+            }
+            __syncthreads();
+            if (active) {
+                const float4 pg = a.tab_cs[r * kMaxTypes + tp];
+                double ds = 0.0, de = 0.0;
+                for (int e = 0; e < ns; e++) {
+                    const float4 q = sp[e];
+                    site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), w, 0.5f * g, pg.x, pg.y, acc, ds, de);
+                }
+                bs += ds * pg.w;
+                beg += de * pg.y;
+            }
+        }
+        if (active) {
+            A.gpart[(size_t)wi * kCompoundBlock + p] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, 0.f);
+            bins_credit(bins, tp, bs, bea, beg);
+        }
+    }
+    if (cur_r >= 0) bins_flush_reset(bins, A.gpar + (size_t)cur_r * kAdjStride);
+}
+
+// ---------------------------------------------------------------------------
+// Compound side, one block per compound: intra-compound pairs (geometric eps,
+// both directions, exclusions of both particles), the bonded terms (dual
+// numbers; per-term coordinate gradients gathered per particle in term order)
+// and the listed-term parts of adj_listed_kernel.  Bonded-constant derivatives
+// are summed over terms.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
+    const StepArgs& a = A.s;
     __shared__ float4 pos[kCompoundBlock];
     __shared__ float4 wv[kCompoundBlock];
-    __shared__ uint32_t nbmask[kCompoundBlock * 4];   // neighbour's exclusions of this compound's particles
     __shared__ double bins[kAdjTypeBins];
     __shared__ float3 gb[kMaxBondsPerCompound * 2];
     __shared__ float3 ga[kMaxAnglesPerCompound * 3];
@@ -461,70 +578,6 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
     wv[p] = make_float4(w.x, w.y, w.z, 0.f);
     D3 acc{0.0, 0.0, 0.0};
     double bs = 0.0, bea = 0.0, beg = 0.0;
-    // (1) listed solvents: "particle <- solvent", geometric eps (Engine.cu:964-974)
-    {
-        const float4 pg = a.tab_cs[r * kMaxTypes + tp];
-        const int ncs = a.enable_solvents ? a.cs_cnt[c] : 0;
-        double ds = 0.0, de = 0.0;
-        for (int j0 = 0; j0 < ncs; j0 += kSolventChunk) {
-            const int ns = min(kSolventChunk, ncs - j0);
-            __syncthreads();
-            for (int k = p; k < ns; k += blockDim.x) {
-                float4 q = a.spos[a.cs_ids[(size_t)c * a.cap_cs + j0 + k]];
-                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
-                sp[k] = q;
-            }
-            __syncthreads();
-            for (int k = 0; k < ns; k++) {
-                const float4 q = sp[k];
-                site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), w, 0.5f * g, pg.x, pg.y, acc, ds, de);
-            }
-        }
-        bs += ds * pg.w;
-        beg += de * pg.y;
-    }
-    // (2) neighbour compounds, arithmetic eps (Engine.cu:653-676); the pair (p, j)
-    // enters p's force unless p excludes j and j's force unless j excludes p
-    const int ncc = a.cc_cnt[c];
-    for (int k = 0; k < ncc; k++) {
-        const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
-        const int nbase = a.comp_off[nb], nn = a.comp_off[nb + 1] - nbase;
-        __syncthreads();
-        if (p < nn) {
-            float4 q = a.cpos[nbase + p];
-            hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
-            sp[p] = q;
-            sw[p] = A.wc[nbase + p];
-            uint32_t m[4] = {0u, 0u, 0u, 0u};
-            for (int e = a.xoff[nbase + p]; e < a.xoff[nbase + p + 1]; e++) {
-                const int2 x = a.xent[e];
-                if (x.x == c) m[x.y >> 5] |= 1u << (x.y & 31);
-            }
-            for (int q4 = 0; q4 < 4; q4++) nbmask[p * 4 + q4] = m[q4];
-        }
-        __syncthreads();
-        if (!active) continue;
-        uint32_t m[4] = {0u, 0u, 0u, 0u};
-        for (int e = a.xoff[gi]; e < a.xoff[gi + 1]; e++) {
-            const int2 x = a.xent[e];
-            if (x.x == nb) m[x.y >> 5] |= 1u << (x.y & 31);
-        }
-        for (int j = 0; j < nn; j++) {
-            const bool in_p = !((m[j >> 5] >> (j & 31)) & 1u);
-            const bool in_j = !((nbmask[j * 4 + (p >> 5)] >> (p & 31)) & 1u);
-            if (!in_p && !in_j) continue;
-            const float4 q = sp[j];
-            const float4 pr = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + ftype(q) * kMaxTypes + tp];
-            const float3 wj = xyz(sw[j]);
-            const float3 u = make_float3((in_p ? w.x : 0.f) - (in_j ? wj.x : 0.f), (in_p ? w.y : 0.f) - (in_j ? wj.y : 0.f),
-                                         (in_p ? w.z : 0.f) - (in_j ? wj.z : 0.f));
-            double ds = 0.0, de = 0.0;
-            site(make_float3(q.x - me.x, q.y - me.y, q.z - me.z), u, 0.5f * g * (float)((int)in_p + (int)in_j), pr.x,
-                 pr.y, acc, ds, de);
-            bs += ds * pr.w;
-            bea += de;
-        }
-    }
     __syncthreads();
     // (3) intra-compound pairs, geometric eps (Engine.cu:678-700), exclusions of both particles
     if (active) {
@@ -591,6 +644,13 @@ __global__ __launch_bounds__(128) void adj_compound_kernel(AdjArgs A) {
             acc.x += x.x;
             acc.y += x.y;
             acc.z += x.z;
+        }
+        // listed terms (neighbour compounds and listed solvents), one part per work item in slot order
+        for (int wi = a.cw_off[c]; wi < a.cw_off[c + 1]; wi++) {
+            const float4 v = A.gpart[(size_t)wi * kCompoundBlock + p];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
         }
         A.gxc[gi] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, 0.f);
         bins_credit(bins, tp, bs, bea, beg);

@@ -86,12 +86,13 @@ struct AdjBufs {
     DBuf<double4> adj;        // [3][N] a_k, a_{k+1}, a_{k+2} in rotation (FP64)
     DBuf<float4> w;           // [N]
     DBuf<float4> gxc, gxs;    // position-gradient parts
+    DBuf<float4> gpart;       // [work item][128] listed-term parts of the compound side
     DBuf<float> cfac, gE;     // [N] dt^2/m, [K][R] energy weights
     DBuf<int> prep;           // [N] system of each particle
     DBuf<double> beta, loss, gpar, efr;
     DBuf<uint32_t> mask;
     DBuf<int> mask_off;
-    int n_cchunks = 1, n_sslices = 1;
+    int n_cchunks = 1, n_sslices = 1, comp_chunk = 1, solv_slice = kSolventChunk;
     bool ready = false;
 };
 
@@ -886,7 +887,7 @@ void adj_setup(lima_engine* E) {
     HIP_TRY(hipMemcpy(mass_h.data(), E->mass.p, mass_h.size() * 4, hipMemcpyDeviceToHost));
     std::vector<float4> cpos_h(std::max(E->NCH, 1));
     if (E->NCH) HIP_TRY(hipMemcpy(cpos_h.data(), E->cpos[E->cur].p, (size_t)E->NCH * 16, hipMemcpyDeviceToHost));
-    int ncmax = 1, nsmax = 1;
+    int maxc = 1, maxs = 1;
     std::vector<int> mask_off(std::max(E->NC, 1), 0);
     int words = 0;
     for (int r = 0; r < E->R; r++) {
@@ -902,16 +903,27 @@ void adj_setup(lima_engine* E) {
             cfac[E->NCH + s] = (float)(dt / (double)mass_h[r * kMaxTypes] * dt);
             prep[E->NCH + s] = r;
         }
-        ncmax = std::max(ncmax, (ri.comp_end - ri.comp_begin + kAdjCompChunk - 1) / kAdjCompChunk);
-        nsmax = std::max(nsmax, (ri.solv_end - ri.solv_begin + kAdjSolvSlice - 1) / kAdjSolvSlice);
+        maxc = std::max(maxc, ri.comp_end - ri.comp_begin);
+        maxs = std::max(maxs, ri.solv_end - ri.solv_begin);
         const int wr = (ri.solv_end - ri.solv_begin + 31) / 32;
         for (int c = ri.comp_begin; c < ri.comp_end; c++) {
             mask_off[c] = words;
             words += wr;
         }
     }
-    B.n_cchunks = ncmax;
-    B.n_sslices = nsmax;
+    // split both all-pairs passes into at least ~2048 blocks: solvent tiles x compound chunks of at most
+    // 8 compounds, and compounds x solvent slices of a multiple of 256
+    const int tiles = std::max(E->n_stiles, 1), comps = std::max(E->NC, 1);
+    int nch = std::max((maxc + kAdjCompChunk - 1) / kAdjCompChunk, (2048 + tiles - 1) / tiles);
+    nch = std::min(nch, maxc);
+    B.comp_chunk = (maxc + nch - 1) / nch;
+    B.n_cchunks = (maxc + B.comp_chunk - 1) / B.comp_chunk;
+    const int nsl = std::max(1, (2048 + comps - 1) / comps);
+    int slice = (maxs + nsl - 1) / nsl;
+    slice = std::max(kSolventChunk, (slice + kSolventChunk - 1) / kSolventChunk * kSolventChunk);
+    B.solv_slice = slice;
+    B.n_sslices = (maxs + slice - 1) / slice;
+    B.gpart.alloc(std::max<size_t>(E->max_slots, 1) * kCompoundBlock);
     B.cfac.upload(cfac);
     B.prep.upload(prep);
     B.mask_off.upload(mask_off);
@@ -1001,10 +1013,14 @@ void adj_frame(lima_engine* E, int j, const float4* X, const float4* rs, const f
     A.cs_mask = B.mask.p;
     A.mask_off = B.mask_off.p;
     A.n_chain = nch;
+    A.gpart = B.gpart.p;
+    A.comp_chunk = B.comp_chunk;
+    A.solv_slice = B.solv_slice;
     const bool sv = solv && ns > 0;
     if (sv) adj_solvent_kernel<<<dim3(E->n_stiles, B.n_cchunks), 256, 0, E->stream>>>(A);
     if (nch > 0) {
         if (sv) adj_chain_solvent_kernel<<<dim3(E->NC, B.n_sslices), kCompoundBlock, 0, E->stream>>>(A);
+        adj_listed_kernel<<<2048, kCompoundBlock, 0, E->stream>>>(A);
         adj_compound_kernel<<<E->NC, kCompoundBlock, 0, E->stream>>>(A);
         if (E->NB > 0) adj_bridge_kernel<<<(E->NB + 63) / 64, 64, 0, E->stream>>>(A);
     }
