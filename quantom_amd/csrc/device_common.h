@@ -136,10 +136,11 @@ LIMA_DEV float lj_scalar(float r2, float A, float B, float& inv_out) {
     return i4 * __builtin_fmaf(B, i3, A);
 }
 
-// FP64 pair energy 4 eps (s^6 - s^3) = -(inv^3 / 6) (A + B inv^3 / 2) of one directed pair
-LIMA_DEV double lj_energy(float r2, float A, float B) {
-    const double i = 1.0 / (double)r2, i3 = i * i * i;
-    return -(i3 / 6.0) * ((double)A + 0.5 * (double)B * i3);
+// pair energy 4 eps (s^6 - s^3) = -(inv^3 / 6) (A + B inv^3 / 2) of one directed pair, from the
+// force's reciprocal; evaluated in FP32 and returned for the FP64 sums
+LIMA_DEV double lj_energy(float inv, float A, float B) {
+    const float i3 = inv * inv * inv;
+    return (double)(-0.16666667f * i3 * __builtin_fmaf(0.5f * B, i3, A));
 }
 
 LIMA_DEV void lj_acc(float px, float py, float pz, float qx, float qy, float qz, float A, float B, float& fx,
@@ -163,7 +164,7 @@ LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, fl
     fx += dx * fs;
     fy += dy * fs;
     fz += dz * fs;
-    e += lj_energy(r2, A, B);
+    e += lj_energy(inv, A, B);
     w += (double)r2 * (double)fs;
 }
 

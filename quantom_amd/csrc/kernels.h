@@ -371,7 +371,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                     const float fs = lj_scalar(r2, pr.x, pr.y, inv);
                     lx += dx * fs; ly += dy * fs; lz += dz * fs;
                     if (ENERGY) {
-                        en += lj_energy(r2, pr.x, pr.y);
+                        en += lj_energy(inv, pr.x, pr.y);
                         vir += (double)r2 * (double)fs;
                     }
                 }
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                         fs = ex ? 0.0f : fs;
                         lx += dx * fs; ly += dy * fs; lz += dz * fs;
                         if (ENERGY && !ex) {
-                            en += lj_energy(r2, pr.x, pr.y);
+                            en += lj_energy(inv, pr.x, pr.y);
                             vir += (double)r2 * (double)fs;
                         }
                     }
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
                 fs = ex ? 0.0f : fs;
                 lx += dx * fs; ly += dy * fs; lz += dz * fs;
                 if (ENERGY && !ex) {
-                    e_intra += lj_energy(r2, pr.x, pr.y);
+                    e_intra += lj_energy(inv, pr.x, pr.y);
                     vir += (double)r2 * (double)fs;
                 }
             }
