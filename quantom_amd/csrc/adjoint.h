@@ -747,9 +747,10 @@ __global__ __launch_bounds__(256) void adj_mask_set_kernel(StepArgs a, uint32_t*
     }
 }
 
-// Per particle: the force residual r = 2 beta (F - F*) of the frame and, when
+// Per particle: the force residual r = 2 beta (F - F*) of the frame (targets as
+// x, y, z triples, or none) and, when
 // `loss` is given, beta |F - F*|^2 summed per system in FP64.
-__global__ __launch_bounds__(256) void adj_residual_kernel(const float4* f, const float4* ftarget, const int* prep,
+__global__ __launch_bounds__(256) void adj_residual_kernel(const float4* f, const float* ftarget, const int* prep,
                                                            const double* beta, float4* rout, double* loss, int n) {
     __shared__ double red[4];
     __shared__ int same;
@@ -760,7 +761,8 @@ __global__ __launch_bounds__(256) void adj_residual_kernel(const float4* f, cons
     if (i < n) {
         const double b = beta[prep[i]];
         const float4 F = f[i];
-        const float4 T = ftarget ? ftarget[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float3 T = ftarget ? make_float3(ftarget[3 * i], ftarget[3 * i + 1], ftarget[3 * i + 2])
+                                 : make_float3(0.f, 0.f, 0.f);
         const float dx = F.x - T.x, dy = F.y - T.y, dz = F.z - T.z;
         const float tb = (float)(2.0 * b);
         rout[i] = make_float4(tb * dx, tb * dy, tb * dz, 0.f);

@@ -82,7 +82,8 @@ struct AdjBufs {
     DBuf<float4> ck;          // [segments][2N]: chain pos, chain prev, solvent pos, solvent prev
     DBuf<int> ck_cc;          // [segments][NC + NC*cap_cc]: compound lists before the segment's rebuild
     DBuf<int> lists;          // [slots][list words]: slot 0 the entry lists, then a segment's periods
-    DBuf<float4> X, Rs, T;    // [segment frames][N]: positions, 2 beta (F - F*), targets F*
+    DBuf<float4> X, Rs;       // [segment frames][N]: positions, 2 beta (F - F*)
+    DBuf<float> Tc, Ts;       // [K][n][3] force targets of the call
     DBuf<double4> adj;        // [3][N] a_k, a_{k+1}, a_{k+2} in rotation (FP64)
     DBuf<float4> w;           // [N]
     DBuf<float4> gxc, gxs;    // position-gradient parts
@@ -945,42 +946,34 @@ void adj_build_mask(lima_engine* E) {
     adj_mask_set_kernel<<<E->NC, 256, 0, E->stream>>>(E->step_args(E->cur), B.mask.p, B.mask_off.p);
 }
 
-// the residuals 2 beta (F - F*) of the frame just stepped into Rs (and the loss when given)
-void adj_residuals(lima_engine* E, float4* rs, const float4* tgt, double* loss, bool solv) {
+// the residuals 2 beta (F - F*) of the frame just stepped into Rs (and the loss when given);
+// targets are the frame's rows of the uploaded chain / solvent target arrays, or none
+void adj_residuals(lima_engine* E, float4* rs, const float* tc, const float* ts, double* loss, bool solv) {
     AdjBufs& B = E->adjb;
     const int nch = E->NCH, ns = E->NS;
     if (nch > 0)
-        adj_residual_kernel<<<(nch + 255) / 256, 256, 0, E->stream>>>(E->cforce.p, tgt, B.prep.p, B.beta.p, rs, loss, nch);
+        adj_residual_kernel<<<(nch + 255) / 256, 256, 0, E->stream>>>(E->cforce.p, tc, B.prep.p, B.beta.p, rs, loss, nch);
     if (ns > 0) {
         if (solv)
-            adj_residual_kernel<<<(ns + 255) / 256, 256, 0, E->stream>>>(E->sforce.p, tgt ? tgt + nch : nullptr,
-                                                                          B.prep.p + nch, B.beta.p, rs + nch, loss, ns);
+            adj_residual_kernel<<<(ns + 255) / 256, 256, 0, E->stream>>>(E->sforce.p, ts, B.prep.p + nch, B.beta.p,
+                                                                          rs + nch, loss, ns);
         else
             HIP_TRY(hipMemsetAsync(rs + nch, 0, (size_t)ns * 16, E->stream));
     }
 }
 
-// targets of frames [k0, k1) into T (float4, chain then solvent per frame)
-void adj_upload_targets(lima_engine* E, const lima_adjoint_spec& sp, int k0, int k1) {
+// all K frames of the targets, once per call, as given (x, y, z per particle)
+void adj_upload_targets(lima_engine* E, const lima_adjoint_spec& sp) {
     AdjBufs& B = E->adjb;
-    const size_t N = (size_t)E->NCH + E->NS;
-    std::vector<float4> h((size_t)(k1 - k0) * N, make_float4(0.f, 0.f, 0.f, 0.f));
-    for (int k = k0; k < k1; k++) {
-        float4* dst = &h[(size_t)(k - k0) * N];
-        if (sp.chain_force_target) {
-            const float* src = sp.chain_force_target + (size_t)k * E->NCH * 3;
-            for (int i = 0; i < E->NCH; i++) dst[i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
-        }
-        if (sp.solvent_force_target) {
-            const float* src = sp.solvent_force_target + (size_t)k * E->NS * 3;
-            for (int i = 0; i < E->NS; i++)
-                dst[E->NCH + i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
-        }
+    const size_t K = sp.n_frames;
+    if (sp.chain_force_target && E->NCH) {
+        ensure(B.Tc, K * E->NCH * 3);
+        HIP_TRY(hipMemcpy(B.Tc.p, sp.chain_force_target, K * E->NCH * 3 * sizeof(float), hipMemcpyHostToDevice));
     }
-    // the previous segment's kernels on the (non-blocking) engine stream may still read T
-    HIP_TRY(hipStreamSynchronize(E->stream));
-    ensure(B.T, h.size());
-    HIP_TRY(hipMemcpy(B.T.p, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice));
+    if (sp.solvent_force_target && E->NS) {
+        ensure(B.Ts, K * E->NS * 3);
+        HIP_TRY(hipMemcpy(B.Ts.p, sp.solvent_force_target, K * E->NS * 3 * sizeof(float), hipMemcpyHostToDevice));
+    }
 }
 
 // one step of the trajectory (the engine's step, optionally with FP64 energies)
@@ -1096,18 +1089,23 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
     }
     HIP_TRY(hipMemcpy(B.beta.p, beta.data(), R * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemsetAsync(B.loss.p, 0, R * 8, E->stream));
-    const bool has_tgt = sp.chain_force_target || sp.solvent_force_target;
+    adj_upload_targets(E, sp);
+    auto tgt_c = [&](int j) -> const float* {
+        return sp.chain_force_target && NCH ? B.Tc.p + (size_t)j * NCH * 3 : nullptr;
+    };
+    auto tgt_s = [&](int j) -> const float* {
+        return sp.solvent_force_target && NS ? B.Ts.p + (size_t)j * NS * 3 : nullptr;
+    };
 
     // ---- forward pass: checkpoints, energies, force loss ----
     save_lists(E, B.lists.p);
     for (int s = 0; s < nseg; s++) {
         const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
         save_state(E, B.ck.p + (size_t)s * 2 * N);
-        if (has_tgt) adj_upload_targets(E, sp, k0, k1);
         for (int j = k0; j < k1; j++) {
             adj_step(E, true);
             d2d(B.efr.p + (size_t)j * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8, E->stream);
-            adj_residuals(E, B.Rs.p, has_tgt ? B.T.p + (size_t)(j - k0) * N : nullptr, B.loss.p, solv);
+            adj_residuals(E, B.Rs.p, tgt_c(j), tgt_s(j), B.loss.p, solv);
             if (j + 1 < K && rebuild_after(j)) {
                 if (s + 1 < nseg && starts[s + 1] == j + 1) save_cc(E, B.ck_cc.p + (size_t)(s + 1) * CW);
                 E->launch_rebuild(E->cur);
@@ -1148,7 +1146,6 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
                 E->launch_rebuild(E->cur);
                 E->since_rebuild = 0;
             }
-            if (has_tgt) adj_upload_targets(E, sp, k0, k1);
             // recompute the segment
             std::vector<int> period_end;
             for (int j = k0; j < k1; j++) {
@@ -1156,8 +1153,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
                 d2d(X, E->cpos[E->cur].p, (size_t)NCH * 16, E->stream);
                 d2d(X + NCH, E->spos[E->cur].p, (size_t)NS * 16, E->stream);
                 adj_step(E, false);
-                adj_residuals(E, B.Rs.p + (size_t)(j - k0) * N, has_tgt ? B.T.p + (size_t)(j - k0) * N : nullptr,
-                              nullptr, solv);
+                adj_residuals(E, B.Rs.p + (size_t)(j - k0) * N, tgt_c(j), tgt_s(j), nullptr, solv);
                 if (j + 1 < k1 && rebuild_after(j)) {
                     save_lists(E, B.lists.p + (size_t)(1 + period_end.size()) * LW);
                     period_end.push_back(j);
