@@ -214,7 +214,7 @@ struct PairDeriv {
 };
 LIMA_DEV PairDeriv pair_deriv(float q, float sig2, float e24) {
     PairDeriv p;
-    p.inv = 1.0f / q;
+    p.inv = __builtin_amdgcn_rcpf(q);   // 1 ulp, as the force's reciprocal
     const float s = sig2 * p.inv;
     p.s3 = s * s * s;
     p.phi = e24 * p.s3 * p.inv * (1.0f - 2.0f * p.s3);
@@ -250,6 +250,7 @@ struct AdjArgs {
     int n_chain;             // NCH: stride of the chain parts
     float4* gpart;           // [work item][128]: listed-term parts of the compound side, per step work item
     int comp_chunk;          // compounds per block of the solvent-side pass (<= kAdjCompChunk)
+    int n_cchunks;           // compound chunks; grid row n_cchunks takes the solvent-solvent pairs
     int solv_slice;          // solvents per block of the chain-side all-pairs pass (multiple of kSolventChunk)
 };
 
@@ -307,7 +308,7 @@ LIMA_DEV float3 neg3(float3 v) { return make_float3(-v.x, -v.y, -v.z); }
 // For every chain particle j of the chunk: the all-pairs site "solvent <- j"
 // (arithmetic eps, Engine.cu:1099-1117) and, when the solvent is in j's
 // compound list, the transpose of "j <- solvent" (geometric eps,
-// Engine.cu:964-974).  Chunk 0 also takes the solvent-solvent pairs, both
+// Engine.cu:964-974).  The grid's last row takes the solvent-solvent pairs, both
 // directions of each symmetric pair (Engine.cu:1119-1128).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(256) void adj_solvent_kernel(AdjArgs A) {
             }
         }
     }
-    if (blockIdx.y == 0) {
+    if (blockIdx.y == A.n_cchunks) {
         const float4 pss = a.tab_ss[r];
         const int cnt = active ? a.ss_cnt[si] : 0;
         for (int k = 0; k < cnt; k++) {
@@ -748,39 +749,34 @@ __global__ __launch_bounds__(256) void adj_mask_set_kernel(StepArgs a, uint32_t*
 }
 
 // Per particle: the force residual r = 2 beta (F - F*) of the frame (targets as
-// x, y, z triples, or none) and, when
-// `loss` is given, beta |F - F*|^2 summed per system in FP64.
+// x, y, z triples, or none) and, when `lacc` is given, beta |F - F*|^2 added to the
+// particle's own FP64 loss accumulator (one owner per entry: no atomics).
 __global__ __launch_bounds__(256) void adj_residual_kernel(const float4* f, const float* ftarget, const int* prep,
-                                                           const double* beta, float4* rout, double* loss, int n) {
-    __shared__ double red[4];
-    __shared__ int same;
+                                                           const double* beta, float4* rout, double* lacc, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int first = blockIdx.x * blockDim.x, last = min(first + (int)blockDim.x, n) - 1;
-    if (threadIdx.x == 0) same = prep[first] == prep[last];
-    double l = 0.0;
-    if (i < n) {
-        const double b = beta[prep[i]];
-        const float4 F = f[i];
-        const float3 T = ftarget ? make_float3(ftarget[3 * i], ftarget[3 * i + 1], ftarget[3 * i + 2])
-                                 : make_float3(0.f, 0.f, 0.f);
-        const float dx = F.x - T.x, dy = F.y - T.y, dz = F.z - T.z;
-        const float tb = (float)(2.0 * b);
-        rout[i] = make_float4(tb * dx, tb * dy, tb * dz, 0.f);
-        l = b * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
-    }
-    if (!loss) return;
+    if (i >= n) return;
+    const double b = beta[prep[i]];
+    const float4 F = f[i];
+    const float3 T = ftarget ? make_float3(ftarget[3 * i], ftarget[3 * i + 1], ftarget[3 * i + 2])
+                             : make_float3(0.f, 0.f, 0.f);
+    const float dx = F.x - T.x, dy = F.y - T.y, dz = F.z - T.z;
+    const float tb = (float)(2.0 * b);
+    rout[i] = make_float4(tb * dx, tb * dy, tb * dz, 0.f);
+    if (lacc) lacc[i] += b * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+}
+
+// per system: the sum of its particles' loss accumulators (chain range, then solvent range)
+__global__ __launch_bounds__(256) void adj_loss_reduce_kernel(const ReplicaInfo* reps, const double* lacc, int nch,
+                                                              double* loss) {
+    __shared__ double red[4];
+    const ReplicaInfo rp = reps[blockIdx.x];
+    double s = 0.0;
+    for (int i = rp.chain_begin + threadIdx.x; i < rp.chain_end; i += blockDim.x) s += lacc[i];
+    for (int i = rp.solv_begin + threadIdx.x; i < rp.solv_end; i += blockDim.x) s += lacc[nch + i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (!same) {
-        if (i < n && l != 0.0) atomicAdd(&loss[prep[i]], l);
-        return;
-    }
-    l = wave_sum(l);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double t = red[0] + red[1] + red[2] + red[3];
-        if (t != 0.0) atomicAdd(&loss[prep[first]], t);
-    }
+    if (threadIdx.x == 0) loss[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // w_k = (dt^2/m) a_{k+1} + r_k

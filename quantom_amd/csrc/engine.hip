@@ -90,7 +90,7 @@ struct AdjBufs {
     DBuf<float4> gpart;       // [work item][128] listed-term parts of the compound side
     DBuf<float> cfac, gE;     // [N] dt^2/m, [K][R] energy weights
     DBuf<int> prep;           // [N] system of each particle
-    DBuf<double> beta, loss, gpar, efr;
+    DBuf<double> beta, loss, gpar, efr, lacc;
     DBuf<uint32_t> mask;
     DBuf<int> mask_off;
     int n_cchunks = 1, n_sslices = 1, comp_chunk = 1, solv_slice = kSolventChunk;
@@ -932,9 +932,10 @@ void adj_setup(lima_engine* E) {
     B.adj.alloc((size_t)3 * std::max(N, 1));
     B.w.alloc(std::max(N, 1));
     B.gxc.alloc((size_t)(1 + B.n_sslices) * std::max(E->NCH, 1));
-    B.gxs.alloc((size_t)B.n_cchunks * std::max(E->NS, 1));
+    B.gxs.alloc((size_t)(B.n_cchunks + 1) * std::max(E->NS, 1));
     B.beta.alloc(E->R);
     B.loss.alloc(E->R);
+    B.lacc.alloc(std::max(N, 1));
     B.gpar.alloc((size_t)E->R * kAdjStride);
     B.ready = true;
 }
@@ -948,15 +949,16 @@ void adj_build_mask(lima_engine* E) {
 
 // the residuals 2 beta (F - F*) of the frame just stepped into Rs (and the loss when given);
 // targets are the frame's rows of the uploaded chain / solvent target arrays, or none
-void adj_residuals(lima_engine* E, float4* rs, const float* tc, const float* ts, double* loss, bool solv) {
+void adj_residuals(lima_engine* E, float4* rs, const float* tc, const float* ts, bool with_loss, bool solv) {
     AdjBufs& B = E->adjb;
     const int nch = E->NCH, ns = E->NS;
     if (nch > 0)
-        adj_residual_kernel<<<(nch + 255) / 256, 256, 0, E->stream>>>(E->cforce.p, tc, B.prep.p, B.beta.p, rs, loss, nch);
+        adj_residual_kernel<<<(nch + 255) / 256, 256, 0, E->stream>>>(E->cforce.p, tc, B.prep.p, B.beta.p, rs,
+                                                                       with_loss ? B.lacc.p : nullptr, nch);
     if (ns > 0) {
         if (solv)
             adj_residual_kernel<<<(ns + 255) / 256, 256, 0, E->stream>>>(E->sforce.p, ts, B.prep.p + nch, B.beta.p,
-                                                                          rs + nch, loss, ns);
+                                                                          rs + nch, with_loss ? B.lacc.p + nch : nullptr, ns);
         else
             HIP_TRY(hipMemsetAsync(rs + nch, 0, (size_t)ns * 16, E->stream));
     }
@@ -1008,9 +1010,10 @@ void adj_frame(lima_engine* E, int j, const float4* X, const float4* rs, const f
     A.n_chain = nch;
     A.gpart = B.gpart.p;
     A.comp_chunk = B.comp_chunk;
+    A.n_cchunks = B.n_cchunks;
     A.solv_slice = B.solv_slice;
     const bool sv = solv && ns > 0;
-    if (sv) adj_solvent_kernel<<<dim3(E->n_stiles, B.n_cchunks), 256, 0, E->stream>>>(A);
+    if (sv) adj_solvent_kernel<<<dim3(E->n_stiles, B.n_cchunks + 1), 256, 0, E->stream>>>(A);
     if (nch > 0) {
         if (sv) adj_chain_solvent_kernel<<<dim3(E->NC, B.n_sslices), kCompoundBlock, 0, E->stream>>>(A);
         adj_listed_kernel<<<2048, kCompoundBlock, 0, E->stream>>>(A);
@@ -1018,7 +1021,7 @@ void adj_frame(lima_engine* E, int j, const float4* X, const float4* rs, const f
         if (E->NB > 0) adj_bridge_kernel<<<(E->NB + 63) / 64, 64, 0, E->stream>>>(A);
     }
     adj_update_kernel<<<(N + 255) / 256, 256, 0, E->stream>>>(B.gxc.p, nch > 0 ? 1 + (sv ? B.n_sslices : 0) : 0, B.gxs.p,
-                                                              sv ? B.n_cchunks : 0, a1, a2, a_out, nch, ns);
+                                                              sv ? B.n_cchunks + 1 : 0, a1, a2, a_out, nch, ns);
 }
 
 struct SavedRun {
@@ -1088,7 +1091,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
         if (sp.energy_weight) alpha[r] = sp.energy_weight[r];
     }
     HIP_TRY(hipMemcpy(B.beta.p, beta.data(), R * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemsetAsync(B.loss.p, 0, R * 8, E->stream));
+    HIP_TRY(hipMemsetAsync(B.lacc.p, 0, (size_t)N * 8, E->stream));
     adj_upload_targets(E, sp);
     auto tgt_c = [&](int j) -> const float* {
         return sp.chain_force_target && NCH ? B.Tc.p + (size_t)j * NCH * 3 : nullptr;
@@ -1105,7 +1108,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
         for (int j = k0; j < k1; j++) {
             adj_step(E, true);
             d2d(B.efr.p + (size_t)j * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8, E->stream);
-            adj_residuals(E, B.Rs.p, tgt_c(j), tgt_s(j), B.loss.p, solv);
+            adj_residuals(E, B.Rs.p, tgt_c(j), tgt_s(j), true, solv);
             if (j + 1 < K && rebuild_after(j)) {
                 if (s + 1 < nseg && starts[s + 1] == j + 1) save_cc(E, B.ck_cc.p + (size_t)(s + 1) * CW);
                 E->launch_rebuild(E->cur);
@@ -1113,6 +1116,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
             }
         }
     }
+    adj_loss_reduce_kernel<<<R, 256, 0, E->stream>>>(E->reps.p, B.lacc.p, NCH, B.loss.p);
     HIP_TRY(hipStreamSynchronize(E->stream));
     int st = E->check_errors();
     std::vector<double> efr((size_t)K * R * kEnergyTerms), floss(R);
@@ -1153,7 +1157,7 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
                 d2d(X, E->cpos[E->cur].p, (size_t)NCH * 16, E->stream);
                 d2d(X + NCH, E->spos[E->cur].p, (size_t)NS * 16, E->stream);
                 adj_step(E, false);
-                adj_residuals(E, B.Rs.p + (size_t)(j - k0) * N, tgt_c(j), tgt_s(j), nullptr, solv);
+                adj_residuals(E, B.Rs.p + (size_t)(j - k0) * N, tgt_c(j), tgt_s(j), false, solv);
                 if (j + 1 < k1 && rebuild_after(j)) {
                     save_lists(E, B.lists.p + (size_t)(1 + period_end.size()) * LW);
                     period_end.push_back(j);
