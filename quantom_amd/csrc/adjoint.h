@@ -230,8 +230,9 @@ LIMA_DEV float3 pair_h(const PairDeriv& p, float3 d, float3 w) {
 // derivatives of  wd * phi + gU * U  with respect to e24 and sigma^2, where U is the
 // full pair energy (e24/6)(s3^2 - s3) scaled by gU
 LIMA_DEV void pair_param(const PairDeriv& p, float wd, float gU, float sig2, float e24, float& de24, float& dsig2) {
-    de24 = wd * p.s3 * (1.0f - 2.0f * p.s3) * p.inv + gU * (p.s3 * p.s3 - p.s3) / 6.0f;
-    dsig2 = (wd * e24 * p.s3 * (3.0f - 12.0f * p.s3) * p.inv + gU * e24 * p.s3 * (2.0f * p.s3 - 1.0f) * 0.5f) / sig2;
+    de24 = wd * p.s3 * (1.0f - 2.0f * p.s3) * p.inv + gU * (p.s3 * p.s3 - p.s3) * (1.0f / 6.0f);
+    dsig2 = (wd * e24 * p.s3 * (3.0f - 12.0f * p.s3) * p.inv + gU * e24 * p.s3 * (2.0f * p.s3 - 1.0f) * 0.5f) *
+            __builtin_amdgcn_rcpf(sig2);
 }
 
 constexpr int kAdjStride = kAdjTypeBins + 4;   // per replica: type bins, then kb, b0, k_theta, theta0
