@@ -187,6 +187,13 @@ int lima_compute_forces(lima_engine* engine, float* chain_forces, float* solvent
 
 /* FP64 energies of the current state, one record per system */
 int lima_get_energies(lima_engine* engine, lima_energies* out);
+/* lima_step with an energy series: the first step and every `every`-th after it
+   also sum the FP64 energies on the device (Analyzer::analyzeEnergy's per-step
+   series, Analyzer.cu:137-207, made in-step).  A record holds the potential
+   terms at the step's positions and the kinetic energy / temperature of the
+   step's displacement (what the thermostat uses).  out: [ceil(n_steps/every)][n_systems].
+   One host synchronisation at the end. */
+int lima_step_observed(lima_engine* engine, int32_t n_steps, int32_t every, lima_energies* out);
 
 /* neighbour lists (NeighborList, Bodies.cuh:203-292) as global ids within a system.
    counts: [n_compounds] / [n_solvents]; ids: stride = the matching capacity. */

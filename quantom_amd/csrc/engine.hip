@@ -1388,6 +1388,23 @@ void save_tune_entry(const std::string& key, const LaunchConfig& c) {
     f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << "\n";
 }
 
+// one system's energy record from the device sums (half of every directed-pair LJ sum)
+void fill_energies(const lima_engine* E, int r, const double* v, float lam, lima_energies& o) {
+    const ReplicaInfo& ri = E->reps_h[r];
+    std::memset(&o, 0, sizeof(o));
+    o.bond = v[kE_bond]; o.angle = v[kE_angle]; o.dihedral = v[kE_dih];
+    o.lj_intra = 0.5 * v[kE_intra]; o.lj_inter = 0.5 * v[kE_inter];
+    o.lj_compound_solvent = 0.5 * v[kE_cs]; o.lj_solvent_compound = 0.5 * v[kE_sc];
+    o.lj_solvent_solvent = 0.5 * v[kE_ss];
+    o.potential = o.bond + o.angle + o.dihedral + o.lj_intra + o.lj_inter + o.lj_compound_solvent +
+                  o.lj_solvent_compound + o.lj_solvent_solvent;
+    o.kinetic = 0.5 * v[kE_kin] / (ri.dt * ri.dt);
+    const double ntot = (double)((ri.chain_end - ri.chain_begin) + (ri.solv_end - ri.solv_begin));
+    o.temperature = ntot > 0 ? v[kE_kin] / (ri.dt * ri.dt) / (3.0 * ntot * 8.3145) : 0.0;
+    o.virial = 0.5 * v[kE_virial];
+    o.thermostat_scalar = lam;
+}
+
 void get3(const std::vector<float4>& v, float* out) {
     for (size_t i = 0; i < v.size(); i++) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
 }
@@ -1566,23 +1583,56 @@ int lima_get_energies(lima_engine* E, lima_energies* out) {
         HIP_TRY(hipMemcpy(e.data(), E->energy.p, e.size() * sizeof(double), hipMemcpyDeviceToHost));
         std::vector<float> lam(E->R);
         HIP_TRY(hipMemcpy(lam.data(), E->lambda_applied.p, E->R * sizeof(float), hipMemcpyDeviceToHost));
-        for (int r = 0; r < E->R; r++) {
-            const double* v = &e[(size_t)r * kEnergyTerms];
-            const ReplicaInfo& ri = E->reps_h[r];
-            lima_energies& o = out[r];
-            std::memset(&o, 0, sizeof(o));
-            o.bond = v[kE_bond]; o.angle = v[kE_angle]; o.dihedral = v[kE_dih];
-            o.lj_intra = 0.5 * v[kE_intra]; o.lj_inter = 0.5 * v[kE_inter];
-            o.lj_compound_solvent = 0.5 * v[kE_cs]; o.lj_solvent_compound = 0.5 * v[kE_sc];
-            o.lj_solvent_solvent = 0.5 * v[kE_ss];
-            o.potential = o.bond + o.angle + o.dihedral + o.lj_intra + o.lj_inter + o.lj_compound_solvent +
-                          o.lj_solvent_compound + o.lj_solvent_solvent;
-            o.kinetic = 0.5 * v[kE_kin] / (ri.dt * ri.dt);
-            const double ntot = (double)((ri.chain_end - ri.chain_begin) + (ri.solv_end - ri.solv_begin));
-            o.temperature = ntot > 0 ? v[kE_kin] / (ri.dt * ri.dt) / (3.0 * ntot * 8.3145) : 0.0;
-            o.virial = 0.5 * v[kE_virial];
-            o.thermostat_scalar = lam[r];
+        for (int r = 0; r < E->R; r++) fill_energies(E, r, &e[(size_t)r * kEnergyTerms], lam[r], out[r]);
+        return E->check_errors();
+    });
+}
+
+int lima_step_observed(lima_engine* E, int32_t n, int32_t every, lima_energies* out) {
+    return guarded(E, [&] {
+        if (n < 0 || every <= 0) throw LimaError(LIMA_E_ARG, "n_steps < 0 or every <= 0");
+        if (!out && n > 0) throw LimaError(LIMA_E_ARG, "null output");
+        HIP_TRY(hipSetDevice(E->device));
+        const int R = E->R;
+        const int nrec = (n + every - 1) / every;
+        std::vector<double> e((size_t)std::max(nrec, 1) * R * kEnergyTerms);
+        std::vector<float> lam((size_t)std::max(nrec, 1) * R);
+        DBuf<double> series;
+        DBuf<float> lser;
+        series.alloc(e.size());
+        lser.alloc(lam.size());
+        int rec = 0;
+        for (int s = 0; s < n;) {
+            // the recorded step: the step itself with its energy sums (Engine::step + the Analyzer's sums)
+            HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
+            E->launch_step(E->cur, true);
+            HIP_TRY(hipMemcpyAsync(series.p + (size_t)rec * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8,
+                                   hipMemcpyDeviceToDevice, E->stream));
+            HIP_TRY(hipMemcpyAsync(lser.p + (size_t)rec * R, E->lambda_applied.p, (size_t)R * 4, hipMemcpyDeviceToDevice,
+                                   E->stream));
+            E->cur ^= 1;
+            E->step++;
+            E->since_rebuild++;
+            if (E->params.nlist_interval > 0 && E->since_rebuild >= E->params.nlist_interval) {
+                E->launch_rebuild(E->cur);
+                E->since_rebuild = 0;
+                E->n_rebuilds++;
+            }
+            rec++;
+            s++;
+            // the plain steps up to the next record (graph replays where the cycle aligns)
+            const int run = std::min(every - 1, n - s);
+            if (run > 0) E->enqueue_steps(run);
+            s += run;
         }
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        if (nrec > 0) {
+            HIP_TRY(hipMemcpy(e.data(), series.p, (size_t)nrec * R * kEnergyTerms * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(lam.data(), lser.p, (size_t)nrec * R * 4, hipMemcpyDeviceToHost));
+        }
+        for (int k = 0; k < nrec; k++)
+            for (int r = 0; r < R; r++)
+                fill_energies(E, r, &e[((size_t)k * R + r) * kEnergyTerms], lam[(size_t)k * R + r], out[(size_t)k * R + r]);
         return E->check_errors();
     });
 }

@@ -30,6 +30,7 @@ EXPORTS = (
     "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
     "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_get_config",
     "lima_autotune", "lima_get_parameters", "lima_set_parameters", "lima_adjoint_run", "lima_record_frames",
+    "lima_step_observed",
 )
 
 
@@ -76,6 +77,7 @@ def load_library() -> C.CDLL:
     L.lima_set_parameters.argtypes = [vp, C.c_int32, dp]
     L.lima_adjoint_run.argtypes = [vp, C.POINTER(CAdjointSpec), dp, dp]
     L.lima_record_frames.argtypes = [vp, C.c_int32, fp, fp, dp]
+    L.lima_step_observed.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(CEnergies)]
     _lib = L
     return L
 
@@ -195,6 +197,16 @@ class Engine:
         arr = (CEnergies * n)()
         self._check(self.L.lima_get_energies(self.h, arr))
         return [arr[k].as_dict() for k in range(n)]
+
+    def step_observed(self, n: int, every: int = 1) -> list:
+        """n steps; the first and every `every`-th step after it record each system's energies.
+        Returns [record][system] dicts (lima_energies fields)."""
+        R = len(self.simulation.systems)
+        nrec = (n + every - 1) // every
+        arr = (CEnergies * max(nrec * R, 1))()
+        self._check(self.L.lima_step_observed(self.h, n, every, arr))
+        self.steps += n
+        return [[arr[k * R + r].as_dict() for r in range(R)] for k in range(nrec)]
 
     # ---- neighbour lists (global ids within the concatenation) ----
     def neighbor_counts(self):
