@@ -68,6 +68,29 @@ public:
         check(lima_get_energies(h_, e.data()));
         return e;
     }
+    // energy series of Analyzer::analyzeEnergy, summed in the step (records on step 0, every, 2*every, ...)
+    std::vector<lima_energies> stepObserved(int n_steps, int every, int n_systems) {
+        std::vector<lima_energies> e((size_t)((n_steps + every - 1) / every) * n_systems);
+        check(lima_step_observed(h_, n_steps, every, e.data()));
+        return e;
+    }
+    // parameter vector of one system: sigma[32], epsilon[32], kb, b0, k_theta, theta_0
+    std::vector<double> parameters(int system) {
+        std::vector<double> v(LIMA_N_PARAMETERS);
+        check(lima_get_parameters(h_, system, v.data()));
+        return v;
+    }
+    void setParameters(int system, const std::vector<double>& v) {
+        if (v.size() != LIMA_N_PARAMETERS) throw Error(LIMA_E_ARG, "expected LIMA_N_PARAMETERS values");
+        check(lima_set_parameters(h_, system, v.data()));
+    }
+    // loss per system and its gradient [n_systems][LIMA_N_PARAMETERS]; the state is left as it was
+    std::vector<double> adjoint(const lima_adjoint_spec& spec, int n_systems, std::vector<double>* grad) {
+        std::vector<double> loss(n_systems);
+        if (grad) grad->assign((size_t)n_systems * LIMA_N_PARAMETERS, 0.0);
+        check(lima_adjoint_run(h_, &spec, loss.data(), grad ? grad->data() : nullptr));
+        return loss;
+    }
     lima_engine* handle() { return h_; }
 
     Int3 timings;
