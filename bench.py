@@ -36,7 +36,8 @@ def workload(name):
     if name == "large":
         return [systems.large()], "large: 10 chains x 40 compounds x 128 + lattice solvent 33/nm^3, 18 nm, 16 types"
     if name == "replicas":
-        return systems.replicas(256), "replicas: 256 systems, chain 128-1024, solvent 500-6000, 3.6-6 nm"
+        return systems.replicas(256), ("replicas: 256 systems, chain 128-1024 in compounds of 64, solvent 500-6000, "
+                                       "5.5-6 nm")
     raise SystemExit(f"unknown workload {name}")
 
 

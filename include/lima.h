@@ -121,7 +121,11 @@ typedef struct lima_system {
 } lima_system;
 
 /* ---- run parameters (compile-time constants of Constants.cuh made runtime) */
-enum { LIMA_THERMOSTAT_NONE = 0, LIMA_THERMOSTAT_BERENDSEN = 1 };
+/* NONE: lambda = 1.  BERENDSEN: every step, lambda = sqrt(1 + dt/tau (T0/T - 1)).
+ * REFERENCE: Engine::hostMaster's cadence (Engine.cu:49-75, 314-393): every 40th
+ * step, from the two-step kinetic energy, lambda = T0/T for one step; both
+ * clamp lambda to [0.9, 1.1]. */
+enum { LIMA_THERMOSTAT_NONE = 0, LIMA_THERMOSTAT_BERENDSEN = 1, LIMA_THERMOSTAT_REFERENCE = 2 };
 enum { LIMA_ACCUM_FP32 = 0, LIMA_ACCUM_FIXED64 = 1 };
 
 typedef struct lima_run_params {

@@ -187,6 +187,16 @@ inline V3<T> integrate(V3<T>& pos, V3<T>& pos_tsub1, const V3<T>& force, double 
     return pos - pos_tsub1;
 }
 
+// LIMAENG::calcKineticEnergy (Engine.cuh:31-44): positions two steps apart, the
+// older re-imaged next to the newer; v = |a - b| * 0.5 / dt and KE = 0.5 m v^2,
+// each narrowed to T.
+template <typename T>
+inline T ref_kinetic(const V3<T>& a, V3<T> b, double mass, double dt, T L) {
+    hyperpos(a, b, L);
+    T vel = (T)((double)((a - b).len() * T(0.5)) / dt);
+    return (T)(0.5 * mass * (double)vel * (double)vel);
+}
+
 // Engine::neighborWithinCutoff (Engine.cu:146-151)
 template <typename T>
 inline bool within(const V3<T>& a, const V3<T>& b, T offset, T cutoff, T L) {
@@ -232,6 +242,8 @@ template <typename T> struct Oracle {
     long step = 0;
     int err = LIMA_OK;
     double last_ke = 0;
+    double ke_ref = 0;   // sum of ref_kinetic over the last advanced step (the reference cadence thermostat)
+    double ref_temperature = 0;   // the last temperature that thermostat evaluated
 
     bool ignore(int p_global, int other_compound, int other_local) const {
         // LJ_Ignores::ignore (Bodies.cuh:191-197), full-width ids
@@ -294,9 +306,9 @@ template <typename T> struct Oracle {
     // is false only forces are produced (state untouched).
     void compound_pass(bool advance, lima_energies* E, double* ke, std::vector<V3<T>>& cx_next) {
         const int ts = 0;  // ATOMTYPE_SOL
-        double e_b = 0, e_a = 0, e_d = 0, e_intra = 0, e_inter = 0, e_cs = 0, vir = 0, kesum = 0;
+        double e_b = 0, e_a = 0, e_d = 0, e_intra = 0, e_inter = 0, e_cs = 0, vir = 0, kesum = 0, kref = 0;
         int local_err = LIMA_OK;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : e_b, e_a, e_d, e_intra, e_inter, e_cs, vir, kesum)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : e_b, e_a, e_d, e_intra, e_inter, e_cs, vir, kesum, kref)
         for (int c = 0; c < nc; c++) {
             const int base = coff[c], n = coff[c + 1] - coff[c];
             std::vector<V3<T>> pos(cx.begin() + base, cx.begin() + base + n);
@@ -371,8 +383,10 @@ template <typename T> struct Oracle {
                 }
             }
             if (!advance) continue;
+            std::vector<V3<T>> older(n);
             for (int p = 0; p < n; p++) {
                 V3<T> prev = cxm[base + p];
+                older[p] = prev;
                 V3<T> disp = integrate(pos[p], prev, force[p], mass(ctype[base + p]), dt, lambda, L);
                 cxm[base + p] = prev;
                 kesum += mass(ctype[base + p]) * ((double)disp.x * disp.x + (double)disp.y * disp.y + (double)disp.z * disp.z);
@@ -382,6 +396,8 @@ template <typename T> struct Oracle {
             apply_pbc(pos[0], L);
             for (int p = 0; p < n; p++) hyperpos(pos[0], pos[p], L);
             for (int p = 0; p < n; p++) cx_next[base + p] = pos[p];
+            // the logged positions (Engine.cu:1013) against those two steps back (Engine.cu:333-337)
+            for (int p = 0; p < n; p++) kref += (double)ref_kinetic(pos[p], older[p], mass(ctype[base + p]), dt, L);
         }
         if (local_err) err = local_err;
         if (E) {
@@ -390,14 +406,15 @@ template <typename T> struct Oracle {
             E->virial += 0.5 * vir;
         }
         if (ke) *ke += kesum;
+        if (advance) ke_ref += kref;
     }
 
     // solventForceKernel (Engine.cu:1070-1161) for every solvent.
     void solvent_pass(bool advance, lima_energies* E, double* ke, std::vector<V3<T>>& sx_next) {
         const int ts = 0;
-        double e_sc = 0, e_ss = 0, vir = 0, kesum = 0;
+        double e_sc = 0, e_ss = 0, vir = 0, kesum = 0, kref = 0;
         int local_err = LIMA_OK;
-#pragma omp parallel for schedule(static) reduction(+ : e_sc, e_ss, vir, kesum)
+#pragma omp parallel for schedule(static) reduction(+ : e_sc, e_ss, vir, kesum, kref)
         for (int s = 0; s < ns; s++) {
             V3<T> sp = sx[s], spm = sxm[s], force;
             double esc = 0, ess = 0, v = 0;
@@ -424,8 +441,10 @@ template <typename T> struct Oracle {
             e_sc += esc; e_ss += ess; vir += v;
             sforce[s] = force;
             if (!advance) continue;
+            const V3<T> older = spm;
             V3<T> disp = integrate(sp, spm, force, mass(ts), dt, lambda, L);
             apply_pbc(sp, L);
+            kref += (double)ref_kinetic(sp, older, mass(ts), dt, L);           // Engine.cu:347-351
             kesum += mass(ts) * ((double)disp.x * disp.x + (double)disp.y * disp.y + (double)disp.z * disp.z);
             if (sp.x != sp.x) {
 #pragma omp critical
@@ -437,6 +456,24 @@ template <typename T> struct Oracle {
         if (local_err && !err) err = local_err;
         if (E) { E->lj_solvent_compound += 0.5 * e_sc; E->lj_solvent_solvent += 0.5 * e_ss; E->virial += 0.5 * vir; }
         if (ke) *ke += kesum;
+        if (advance) ke_ref += kref;
+    }
+
+    // Engine::hostMaster's cadence (Engine.cu:49-75) with handleBoxtemp (Engine.cu:362-393)
+    // and getBoxTemperature (Engine.cu:314-359): with s steps done, s % 40 == 0 evaluates
+    // T = avg KE * 2 / (3 R) from the two-step kinetic energies; when s > 20 the scalar
+    // T0 / T, clamped to [0.9, 1.1], applies to the next step only (reset at s % 40 == 1).
+    void update_lambda_reference(long s) {
+        lambda = T(1);
+        if (s % 40 != 0) return;
+        double avg = ke_ref / (double)(nchain + ns);
+        float temp = (float)(avg * 2.f / (3.f * GAS_R));
+        temp = temp == 0.f ? 1.f : temp;
+        ref_temperature = temp;
+        float sc = (float)T0 / temp;
+        sc = std::min(sc, 1.1f);
+        sc = std::max(sc, 0.9f);
+        if (s > 20) lambda = (T)sc;
     }
 
     void update_lambda(double kesum) {
@@ -455,6 +492,7 @@ template <typename T> struct Oracle {
     int step_once() {
         std::vector<V3<T>> cx_next(cx.size()), sx_next(sx.size());
         double ke = 0;
+        ke_ref = 0;
         bridge_pass(nullptr);
         compound_pass(true, nullptr, &ke, cx_next);
         if (enable_solvents) solvent_pass(true, nullptr, &ke, sx_next);
@@ -462,8 +500,9 @@ template <typename T> struct Oracle {
         cx.swap(cx_next);                                                      // Engine.cu:425-432
         sx.swap(sx_next);
         last_ke = ke;
-        update_lambda(ke);
         step++;
+        if (thermostat == LIMA_THERMOSTAT_REFERENCE) update_lambda_reference(step);
+        else update_lambda(ke);
         if (nlist_interval > 0 && step % nlist_interval == 0) update_neighbors();
         return err;
     }
@@ -758,6 +797,12 @@ int oracle_get_energies(void* h, lima_energies* out) {
 double oracle_get_lambda(void* h) {
     double v = 0;
     DISPATCH(h, v = (double)o.lambda);
+    return v;
+}
+
+double oracle_get_ref_temperature(void* h) {
+    double v = 0;
+    DISPATCH(h, v = o.ref_temperature);
     return v;
 }
 

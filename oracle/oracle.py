@@ -40,6 +40,8 @@ def lib(fast: bool = False) -> C.CDLL:
         L.oracle_get_lambda.restype = C.c_double
         L.oracle_get_lambda.argtypes = [C.c_void_p]
         L.oracle_set_lambda.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_get_ref_temperature.restype = C.c_double
+        L.oracle_get_ref_temperature.argtypes = [C.c_void_p]
         L.oracle_get_step.restype = C.c_long
         L.oracle_get_step.argtypes = [C.c_void_p]
         L.oracle_get_neighbor_counts.argtypes = [C.c_void_p, C.c_int, ip]
@@ -142,6 +144,11 @@ class Oracle:
     @lam.setter
     def lam(self, v: float):
         self.L.oracle_set_lambda(self.h, v)
+
+    @property
+    def ref_temperature(self) -> float:
+        """The temperature the reference-cadence thermostat last evaluated."""
+        return self.L.oracle_get_ref_temperature(self.h)
 
     @property
     def step_count(self) -> int:

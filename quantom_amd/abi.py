@@ -26,6 +26,7 @@ STATUS_NAMES = {0: "LIMA_OK", 1: "LIMA_E_HIP", 2: "LIMA_E_CAPACITY", 3: "LIMA_E_
 
 THERMOSTAT_NONE = 0
 THERMOSTAT_BERENDSEN = 1
+THERMOSTAT_REFERENCE = 2   # Engine::hostMaster cadence: T0/T every 40th step, for one step
 ACCUM_FP32 = 0
 ACCUM_FIXED64 = 1
 

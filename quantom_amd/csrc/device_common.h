@@ -99,6 +99,17 @@ LIMA_DEV float dist2_reimaged(float ax, float ay, float az, float bx, float by, 
     return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
 }
 
+// LIMAENG::calcKineticEnergy (Engine.cuh:31-44): positions two steps apart,
+// the older one re-imaged next to the newer; v = |a - b| * 0.5f / dt narrowed to
+// FP32, KE = 0.5 m v^2 in FP64 narrowed to FP32.
+LIMA_DEV float ref_kinetic(float ax, float ay, float az, float bx, float by, float bz, double mass, double dt, float L,
+                           float half) {
+    hyperpos(ax, ay, az, bx, by, bz, L, half);
+    const float d = exact_len(fsub(ax, bx), fsub(ay, by), fsub(az, bz));
+    const float vel = (float)((double)fmul(d, 0.5f) / dt);
+    return (float)(0.5 * mass * (double)vel * (double)vel);
+}
+
 // integratePosition (Engine.cu:849-869); returns the displacement after rescale.
 LIMA_DEV void integrate(float& px, float& py, float& pz, float& qx, float& qy, float& qz, float fx, float fy,
                         float fz, double mts, float lambda, float L, float half, float& dx, float& dy, float& dz) {

@@ -150,6 +150,7 @@ struct lima_engine {
     DBuf<SCWork> scwork;
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
+    DBuf<long long> step_count;   // per replica: steps done (the reference cadence thermostat reads it)
     DBuf<uint32_t> err;
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
     DBuf<float4> ss_part, ss_img;
@@ -239,6 +240,7 @@ struct lima_engine {
         a.lambda = lambda.p;
         a.ke_comp = ke_comp.p;
         a.ke_stile = ke_stile.p;
+        a.ke_ref = params.thermostat == LIMA_THERMOSTAT_REFERENCE ? 1 : 0;
         a.energy = energy.p;
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
@@ -275,7 +277,7 @@ struct lima_engine {
     }
 
     // solvent list row stride: capacity rounded up to 4 ints (16-byte aligned rows)
-    int ss_ld() const { return (params.max_solvent_neighbors + 3) & ~3; }
+    int ss_ld() const { return (params.max_solvent_neighbors + kListChunk - 1) / kListChunk * kListChunk; }
 
     // ---- launches ----
     size_t chunk_lds_bytes() const { return (size_t)chunk_cap * (sizeof(float4) + sizeof(float)); }
@@ -347,7 +349,7 @@ struct lima_engine {
         // s2: bridge carry-in forces, then the listed solvent-solvent LJ
         if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_bridge, s2));
-        if (solv) solvent_listed_kernel<ENERGY><<<n_stiles, 256, 0, s2>>>(a);
+        if (solv) solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles), 256, 0, s2>>>(a, n_stiles);
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s1: compound listed LJ, then bonded + intra + integration of the compounds
         if (NC > 0) {
@@ -379,7 +381,7 @@ struct lima_engine {
         if (!params.enable_solvents && NS > 0)
             HIP_TRY(hipMemcpyAsync(spos[parity ^ 1].p, spos[parity].p, NS * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
-                                                temperature.p, params.thermostat == LIMA_THERMOSTAT_BERENDSEN ? 1 : 0);
+                                                temperature.p, step_count.p, params.thermostat);
     }
 
     // the rebuild forks too: compound lists on s1, the solvent cell pipeline on the main stream
@@ -402,7 +404,7 @@ struct lima_engine {
             nl_cell_scan_kernel<<<1, 1024, 0, stream>>>(a);
             nl_cell_scatter_kernel<<<(NS + 255) / 256, 256, 0, stream>>>(a);
             nl_cell_sort_kernel<<<n_cells, 256, 0, stream>>>(a);
-            nl_solvent_solvent_kernel<<<n_cells, kSolventListThreads, 0, stream>>>(a);
+            nl_solvent_solvent_kernel<<<xcd_grid(n_cells), kSolventListThreads, 0, stream>>>(a);
         }
         HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
         if (NC > 0) nl_work_kernel<<<1, 1024, 0, stream>>>(a);
@@ -447,7 +449,13 @@ struct lima_engine {
         graph_steps = n;
     }
 
+    // the device step counters follow the host's count (set before any stepping call enqueues)
+    void sync_step_count() {
+        set_step_count_kernel<<<(R + 63) / 64, 64, 0, stream>>>(step_count.p, R, step);
+    }
+
     void enqueue_steps(int n) {
+        sync_step_count();
         const int cyc = cycle_steps();
         while (n > 0) {
             if (cfg.use_graph && params.nlist_interval > 0 && cur == 0 && since_rebuild == 0 && n >= cyc) {
@@ -538,6 +546,8 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     std::vector<int> cell_rep_h;
     if (nsys <= 0) throw LimaError(LIMA_E_ARG, "n_systems must be >= 1");
     if (!(P.cutoff > 0)) throw LimaError(LIMA_E_ARG, "cutoff must be > 0");
+    if (P.thermostat < LIMA_THERMOSTAT_NONE || P.thermostat > LIMA_THERMOSTAT_REFERENCE)
+        throw LimaError(LIMA_E_ARG, "unknown thermostat");
     if (P.max_compound_neighbors <= 0 || P.max_compound_solvents <= 0 || P.max_solvent_neighbors <= 0)
         throw LimaError(LIMA_E_ARG, "capacities must be > 0");
     std::vector<float4> cpos, cprev, spos, sprev;
@@ -775,7 +785,9 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->seg_cnt.alloc(NCx * E->max_segs); E->seg_cnt.zero();
     E->cs_ids.alloc(NCx * P.max_compound_solvents);
     E->ss_cnt.alloc(NSx); E->ss_cnt.zero();
-    E->ss_ids.alloc(NSx * E->ss_ld());
+    if ((NSx + 256) * (size_t)E->ss_ld() >= (size_t(1) << 32))
+        throw LimaError(LIMA_E_ARG, "solvent lists exceed 2^32 entries (n_solvents x max_solvent_neighbors)");
+    E->ss_ids.alloc((NSx + 256) * E->ss_ld());   // spare rows: solvent_listed reads whole 64-row groups
     E->cw_off.alloc(NCx + 1); E->cw_off.zero();
     E->cw_total.alloc(1); E->cw_total.zero();
     E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
@@ -792,6 +804,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->ke_comp.alloc(NCx); E->ke_comp.zero();
     E->ke_stile.alloc(std::max(E->n_stiles, 1)); E->ke_stile.zero();
     E->temperature.alloc(nsys); E->temperature.zero();
+    E->step_count.alloc(nsys); E->step_count.zero();
     E->energy.alloc((size_t)nsys * kEnergyTerms); E->energy.zero();
     E->err.alloc(4); E->err.zero();
     E->cell_cnt.alloc(E->n_cells); E->cell_start.alloc(E->n_cells + 1); E->cell_cursor.alloc(E->n_cells);
@@ -1222,6 +1235,11 @@ int record_frames(lima_engine* E, int K, float* cf, float* sf, double* pot) {
     const SavedRun saved{E->cur, E->step, E->since_rebuild, E->n_rebuilds};
     save_state(E, B.ck.p);
     save_lists(E, B.lists.p);
+    // the thermostat scalars are part of the state handed back
+    std::vector<float> lam(R), lam_applied(R);
+    HIP_TRY(hipMemcpy(lam.data(), E->lambda.p, R * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lam_applied.data(), E->lambda_applied.p, R * sizeof(float), hipMemcpyDeviceToHost));
+    E->sync_step_count();
     std::vector<float4> hc(std::max(NCH, 1)), hs(std::max(NS, 1));
     std::vector<double> e((size_t)R * kEnergyTerms);
     int st = LIMA_OK;
@@ -1261,6 +1279,8 @@ int record_frames(lima_engine* E, int K, float* cf, float* sf, double* pot) {
     load_state(E, B.ck.p);
     load_lists(E, B.lists.p);
     HIP_TRY(hipStreamSynchronize(E->stream));
+    HIP_TRY(hipMemcpy(E->lambda.p, lam.data(), R * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->lambda_applied.p, lam_applied.data(), R * sizeof(float), hipMemcpyHostToDevice));
     E->step = saved.step;
     E->since_rebuild = saved.since;
     E->n_rebuilds = saved.nreb;
@@ -1602,6 +1622,7 @@ int lima_step_observed(lima_engine* E, int32_t n, int32_t every, lima_energies* 
         series.alloc(e.size());
         lser.alloc(lam.size());
         int rec = 0;
+        E->sync_step_count();
         for (int s = 0; s < n;) {
             // the recorded step: the step itself with its energy sums (Engine::step + the Analyzer's sums)
             HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
@@ -1766,7 +1787,7 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
             });
         }
         if (E->params.enable_solvents && E->NS > 0)
-            out8[6] = timeit([&] { solvent_listed_kernel<false><<<E->n_stiles, 256, 0, E->stream>>>(a); });
+            out8[6] = timeit([&] { solvent_listed_kernel<false><<<xcd_grid(E->n_stiles), 256, 0, E->stream>>>(a, E->n_stiles); });
         if (E->NC > 0) {
             const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
             out6[1] = timeit([&] {

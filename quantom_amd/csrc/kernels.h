@@ -84,6 +84,7 @@ struct StepArgs {
     const float* lambda;          // [R] applied this step
     double* ke_comp;              // [n_comp]
     double* ke_stile;             // [n_stiles]
+    int ke_ref;                   // 1: the thermostat sums read the reference's two-step kinetic energy
     // energies: [R*kEnergyTerms]
     double* energy;
     uint32_t* err;                // [4]
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
     if (FIXED) force = f3(from_fixed(fx_i), from_fixed(fy_i), from_fixed(fz_i));
     if (active) a.cforce[gi] = make_float4(force.x, force.y, force.z, 0.f);
     if (active && exact_len(force.x, force.y, force.z) > 2e12f) atomicOr(a.err, kErrCriticalForce);   // Engine.cu:1050
-    double ke = 0.0;
+    double ke = 0.0, kt = 0.0;   // one-step m|dx|^2; what the thermostat sums
     const double m = (double)a.mass[r * kMaxTypes + ftype(me)];
     float4 pv = a.cprev[gi];
     if (ADVANCE) {
@@ -566,20 +567,28 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
         float zx = pbc1(z.x, L), zy = pbc1(z.y, L), zz = pbc1(z.z, L);
         hyperpos(zx, zy, zz, px, py, pz, L, half);
         if (active) a.cpos_next[gi] = make_float4(px, py, pz, me.w);
+        // the logged position of this step against the one two steps back (Engine.cu:333-337)
+        kt = !active ? 0.0 : a.ke_ref ? (double)ref_kinetic(px, py, pz, pv.x, pv.y, pv.z, m, rp.dt, L, half) : ke;
     } else if (active) {
         float qx = pv.x, qy = pv.y, qz = pv.z;
         hyperpos(me.x, me.y, me.z, qx, qy, qz, L, half);
         float dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
         ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
     }
-    // kinetic-energy partial of this compound (FP64, fixed order)
-    double ks = wave_sum(ke);
-    if ((p & 63) == 0) red[p >> 6] = ks;
-    __syncthreads();
-    if (p == 0) {
-        double tot = red[0] + (blockDim.x > 64 ? red[1] : 0.0);
-        if (ADVANCE) a.ke_comp[c] = tot;
-        if (ENERGY) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
+    // kinetic-energy partials of this compound (FP64, fixed order)
+    if (ADVANCE) {
+        double ks = wave_sum(kt);
+        __syncthreads();
+        if ((p & 63) == 0) red[p >> 6] = ks;
+        __syncthreads();
+        if (p == 0) a.ke_comp[c] = red[0] + (blockDim.x > 64 ? red[1] : 0.0);
+    }
+    if (ENERGY) {
+        double ks = wave_sum(ke);
+        __syncthreads();
+        if ((p & 63) == 0) red[p >> 6] = ks;
+        __syncthreads();
+        if (p == 0) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, red[0] + (blockDim.x > 64 ? red[1] : 0.0));
     }
     if (ENERGY) {
         block_atomic_energy(eb, a.energy + r * kEnergyTerms + kE_bond);
@@ -597,42 +606,101 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
 // reference mutates it; the solvent-solvent LJ local sum and the final self
 // image are written for the integrator.  One thread per solvent; it runs
 // concurrently with the all-pairs kernel.
+//
+// The lists are rows of ss_ld ids (a multiple of kListChunk).  A wavefront reads
+// the rows of its 64 solvents kListChunk entries at a time, four rows per load
+// instruction (one 64-byte segment each), and transposes them through its own
+// LDS stage; each lane then walks its row from LDS.  Every list byte is read
+// from HBM once, in whole segments, which leaves L2 to the position gathers.
+// Tiles are taken in XCD-contiguous order (xcd_item), so the tiles of one
+// replica share one L2.
 // ---------------------------------------------------------------------------
+constexpr int kListChunk = 16;                 // list entries per lane per stage
+constexpr int kListStride = kListChunk + 4;    // LDS row stride in ints (16-byte aligned, conflict-free b128 reads)
+constexpr int kXcds = 8;
+
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD
+// b % 8).  With the grid padded to a multiple of 8, XCD x takes the contiguous
+// item range [x * per, (x + 1) * per).
+LIMA_DEV int xcd_item(int b, int nblocks) {
+    const int per = nblocks / kXcds;
+    return (b % kXcds) * per + b / kXcds;
+}
+inline int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
+
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a) {
-    const int4 tile = a.stiles[blockIdx.x];
+__global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a, int n_tiles) {
+    // the per-wave list stages; the replay's particle-0 stage aliases them (used before the lists)
+    __shared__ __align__(16) int stage[4 * 64 * kListStride];
+    static_assert(sizeof(stage) >= kReplayStage * sizeof(float4), "replay stage must fit the list stage");
+    const int t = xcd_item(blockIdx.x, gridDim.x);
+    if (t >= n_tiles) return;
+    const int4 tile = a.stiles[t];
     const int r = tile.x;
     const ReplicaInfo rp = a.reps[r];
     const float L = rp.L, half = rp.half;
     const int s = tile.y + threadIdx.x;
     const bool active = s < tile.z;
     const int si = active ? s : tile.y;
-    __shared__ float4 p0s[kReplayStage];
     const float4 me = a.spos[si];
     float sx = me.x, sy = me.y, sz = me.z;
-    replay_images<1>(a, rp.comp_begin, rp.comp_end, p0s, &sx, &sy, &sz, L, half);   // Engine.cu:1111
+    replay_images<1>(a, rp.comp_begin, rp.comp_end, reinterpret_cast<float4*>(stage), &sx, &sy, &sz, L, half);   // Engine.cu:1111
+    __syncthreads();   // the stage is reused for the lists below
     const float4 pv = a.tab_ss[r];
     const float2 pss = lj_ab(pv.x, pv.y);
     float lx = 0.f, ly = 0.f, lz = 0.f;
     double en = 0, vir = 0;
     const int cnt = active ? a.ss_cnt[si] : 0;
-    const int* ids = a.ss_ids + (size_t)si * a.ss_ld;   // 16-byte aligned rows
-    int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-        const int4 i4 = *reinterpret_cast<const int4*>(ids + k);
-        const float4 q[4] = {a.spos[i4.x], a.spos[i4.y], a.spos[i4.z], a.spos[i4.w]};
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int* st = stage + wid * 64 * kListStride;
+    const int wrow0 = tile.y + wid * 64;        // the wave's first solvent (rows beyond tile.z have count 0)
+    int wmax = cnt;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    const int sub = lane >> 4, col = lane & (kListChunk - 1);
+    // this lane loads entry k0 + col of rows 4 i + sub (i < 16); the list buffer has 256 spare
+    // rows, so rows past the tile (count 0) stay inside it
+    int rcs[16];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
-            if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
-            else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
+    for (int i = 0; i < 16; i++) rcs[i] = __shfl(cnt, 4 * i + sub, 64);
+    // 32-bit element offsets (the buffer is under 2^32 bytes, checked at creation) keep one
+    // offset register per load instead of a 64-bit address each
+    const unsigned rbase = (unsigned)(wrow0 + sub) * (unsigned)a.ss_ld;
+    const unsigned rstep = 4u * (unsigned)a.ss_ld;
+    for (int k0 = 0; k0 < wmax; k0 += kListChunk) {
+        // cooperative, coalesced read of 64 row segments, all loads in flight at once; a row
+        // already finished re-reads its entry 0 (no new line) instead of its unused tail
+        int v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            v[i] = a.ss_ids[rbase + (unsigned)i * rstep + (unsigned)(k0 + col < rcs[i] ? k0 + col : 0)];
+#pragma unroll
+        for (int i = 0; i < 16; i++) st[(4 * i + sub) * kListStride + col] = v[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int n = min(kListChunk, cnt - k0);   // this lane's entries in the segment (may be <= 0)
+        const int* mine = st + lane * kListStride;
+        int k = 0;
+        for (; k + 4 <= n; k += 4) {
+            const int4 i4 = *reinterpret_cast<const int4*>(mine + k);
+            const float4 q[4] = {a.spos[i4.x], a.spos[i4.y], a.spos[i4.z], a.spos[i4.w]};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
+                if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
+                else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
+            }
         }
-    }
-    for (; k < cnt; k++) {
-        const float4 q = a.spos[ids[k]];
-        hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-        if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
-        else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
+        for (; k < n; k++) {
+            const float4 q = a.spos[mine[k]];
+            hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
+            if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
+            else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
+        }
+        // every lane is done with the stage before the next segment overwrites it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (active) {
         a.ss_part[s] = make_float4(lx, ly, lz, 0.f);
@@ -678,7 +746,7 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
     }
     if (active) a.sforce[s] = make_float4(fx, fy, fz, 0.f);
     const double m = (double)a.mass[r * kMaxTypes + 0];
-    double ke = 0.0;
+    double ke = 0.0, kt = 0.0;   // one-step m|dx|^2; what the thermostat sums
     float4 pv = a.sprev[si];
     if (ADVANCE) {
         float qx = pv.x, qy = pv.y, qz = pv.z, dx, dy, dz;
@@ -690,6 +758,8 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
             a.spos_next[s] = make_float4(sx, sy, sz, 0.f);
             a.sprev[s] = make_float4(qx, qy, qz, 0.f);
             ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+            // the logged position of this step against the one two steps back (Engine.cu:347-351)
+            kt = a.ke_ref ? (double)ref_kinetic(sx, sy, sz, pv.x, pv.y, pv.z, m, rp.dt, L, half) : ke;
         }
     } else if (active) {
         float qx = pv.x, qy = pv.y, qz = pv.z;
@@ -697,26 +767,47 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
         float dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
         ke = m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
     }
-    double ks = wave_sum(ke);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ks;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double tot = 0;
-        for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
-        if (ADVANCE) a.ke_stile[tile.w] = tot;
-        if (ENERGY) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
+    if (ADVANCE) {
+        double ks = wave_sum(kt);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ks;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0;
+            for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
+            a.ke_stile[tile.w] = tot;
+        }
+    }
+    if (ENERGY) {
+        double ks = wave_sum(ke);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ks;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0;
+            for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
+            atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Thermostat (Engine::handleBoxtemp, Engine.cu:362-393, made a device FP64
-// reduction): per replica, T = sum m|dx|^2 / dt^2 / (3 N R); Berendsen
-// lambda = sqrt(1 + dt/tau (T0/T - 1)) with the reference's [0.9, 1.1] clamp
-// and T=0 guard.  One wavefront per replica, fixed summation order.
+// reduction).  One wavefront per replica, fixed summation order.
+//   mode 1 (Berendsen): T = sum m|dx|^2 / dt^2 / (3 N R) every step;
+//     lambda = sqrt(1 + dt/tau (T0/T - 1)), clamped to [0.9, 1.1], T=0 guard.
+//   mode 2 (the reference's cadence, Engine.cu:49-75, 314-393): after step k,
+//     with s = k + 1 steps done, s % 40 == 0 evaluates T from the per-particle
+//     FP32 kinetic energies of the positions two steps apart,
+//     T = (float)(sum / N * 2 / (3 * 8.3145)), and, when s > 20, sets
+//     lambda = clamp((float)T0 / T, 0.9, 1.1) for step s only; every other
+//     step runs with lambda = 1.  step_count[r] counts the steps done.
 // ---------------------------------------------------------------------------
+constexpr int kThermostatEvery = 40;   // STEPS_PER_THERMOSTAT (Constants.cuh:82)
+
 __global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps, int n_rep, const double* ke_comp,
                                                       const double* ke_stile, float* lambda, float* lambda_applied,
-                                                      double* temperature, int mode) {
+                                                      double* temperature, long long* step_count, int mode) {
     const int r = blockIdx.x;
     if (r >= n_rep) return;
     const ReplicaInfo rp = reps[r];
@@ -726,19 +817,40 @@ __global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps,
     s = wave_sum(s);
     if (threadIdx.x == 0) {
         lambda_applied[r] = lambda[r];
+        const long long done = step_count[r] + 1;
+        step_count[r] = done;
         const double ntot = (double)((rp.chain_end - rp.chain_begin) + (rp.solv_end - rp.solv_begin));
-        double temp = ntot > 0 ? s / (rp.dt * rp.dt) / (3.0 * ntot * 8.3145) : 0.0;
-        temperature[r] = temp;
         float lam = 1.0f;
-        if (mode == 1) {
-            double t = temp == 0.0 ? 1.0 : temp;
-            double l = sqrt(1.0 + rp.dt / rp.tau * ((double)rp.T0 / t - 1.0));
-            l = l < 1.1 ? l : 1.1;
-            l = l > 0.9 ? l : 0.9;
-            lam = (float)l;
+        if (mode == 2) {
+            if (done % kThermostatEvery == 0) {
+                const double avg = ntot > 0 ? s / ntot : 0.0;
+                float temp = (float)(avg * 2.0f / (3.0f * 8.3145));
+                temp = temp == 0.f ? 1.f : temp;
+                temperature[r] = temp;
+                float sc = rp.T0 / temp;
+                sc = fminf(sc, 1.1f);
+                sc = fmaxf(sc, 0.9f);
+                if (done > 20) lam = sc;
+            }
+        } else {
+            const double temp = ntot > 0 ? s / (rp.dt * rp.dt) / (3.0 * ntot * 8.3145) : 0.0;
+            temperature[r] = temp;
+            if (mode == 1) {
+                double t = temp == 0.0 ? 1.0 : temp;
+                double l = sqrt(1.0 + rp.dt / rp.tau * ((double)rp.T0 / t - 1.0));
+                l = l < 1.1 ? l : 1.1;
+                l = l > 0.9 ? l : 0.9;
+                lam = (float)l;
+            }
         }
         lambda[r] = lam;
     }
+}
+
+// sets every replica's step counter (graph-safe: a kernel, not a memset)
+__global__ void set_step_count_kernel(long long* step_count, int n_rep, long long v) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_rep) step_count[r] = v;
 }
 
 // ===========================================================================
@@ -974,24 +1086,27 @@ __global__ __launch_bounds__(256) void nl_cell_sort_kernel(NlistArgs a) {
 // thread tests one home solvent against all candidates.  The pair predicate is
 // evaluated with the lower id as the static point, as the reference's add loop
 // does (Engine.cu:262-264), and compared as d^2 < bound(C + C).
-constexpr int kCellCandidates = 4096;
+// 4032 staged candidates (79 KB with the cell tables) let two blocks share a CU.
+constexpr int kCellCandidates = 4032;
 
 LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, float bound) {
     const bool lo_self = s < j;
     const float ax = lo_self ? me.x : q.x, ay = lo_self ? me.y : q.y, az = lo_self ? me.z : q.z;
     const float bx = lo_self ? q.x : me.x, by = lo_self ? q.y : me.y, bz = lo_self ? q.z : me.z;
-    return (j != s) && dist2_reimaged(ax, ay, az, bx, by, bz, L, half) < bound;
+    return (j != s) & (dist2_reimaged(ax, ay, az, bx, by, bz, L, half) < bound);
 }
 
-// 1024 threads: the 80 KB candidate stage leaves one block per CU, so the block
-// itself must carry the waves that hide LDS and ballot latency (16 per CU)
+// 1024 threads: each block carries 16 waves to hide LDS and ballot latency, and
+// two blocks fit one CU
 constexpr int kSolventListThreads = 1024;
 
 __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel(NlistArgs a) {
     __shared__ float4 cand_pos[kCellCandidates];
     __shared__ int cand_id[kCellCandidates];
     __shared__ int cell_b[27], cell_n[27], cell_off[28];
-    const int cell = blockIdx.x;
+    // cells of one replica are contiguous: XCD-contiguous order keeps a replica's positions in one L2
+    const int cell = xcd_item(blockIdx.x, gridDim.x);
+    if (cell >= a.n_cells) return;
     const int h0 = a.cell_start[cell], h1 = a.cell_start[cell + 1];
     if (h0 == h1) return;
     const ReplicaInfo rp = a.reps[a.cell_rep[cell]];
@@ -1039,22 +1154,30 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
         }
         __syncthreads();
         // each wavefront takes home solvents (staged with the home cell) in turn; its lanes test
-        // 64 candidates at a time and append the members in candidate order (ballot compaction)
+        // 128 candidates per pass (two independent halves, no branches) and append the members
+        // in candidate order (ballot compaction)
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        const unsigned long long below = (1ull << lane) - 1ull;
         const int home0 = cell_off[k_home], nhome = h1 - h0;
         for (int h = wid; h < nhome; h += nw) {
             const int s = cand_id[home0 + h];
             const float4 me = cand_pos[home0 + h];
             int* row = a.ss_ids + (size_t)s * a.ss_ld;
             int count = 0;
-            for (int e0 = 0; e0 < total; e0 += 64) {
-                const int e = e0 + lane;
-                const int j = e < total ? cand_id[e] : s;
-                const bool member = e < total && ss_member(s, me, j, cand_pos[e < total ? e : 0], L, half, bound);
-                const unsigned long long ball = __ballot(member);
-                const int pos = count + __popcll(ball & ((1ull << lane) - 1ull));
-                if (member && pos < a.cap_ss) row[pos] = j;
-                count += __popcll(ball);
+            for (int e0 = 0; e0 < total; e0 += 128) {
+                const int ea = e0 + lane, eb = ea + 64;
+                const int ia = ea < total ? ea : 0, ib = eb < total ? eb : 0;
+                const int ja = cand_id[ia], jb = cand_id[ib];
+                const float4 qa = cand_pos[ia], qb = cand_pos[ib];
+                const bool ma = (ea < total) & ss_member(s, me, ja, qa, L, half, bound);
+                const bool mb = (eb < total) & ss_member(s, me, jb, qb, L, half, bound);
+                const unsigned long long ba = __ballot(ma), bb = __ballot(mb);
+                const int na = __popcll(ba);
+                const int pa = count + __popcll(ba & below);
+                const int pb = count + na + __popcll(bb & below);
+                if (ma & (pa < a.cap_ss)) row[pa] = ja;
+                if (mb & (pb < a.cap_ss)) row[pb] = jb;
+                count += na + __popcll(bb);
             }
             if (lane == 0) {
                 if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }

@@ -194,18 +194,37 @@ def _compound_sphere(p: np.ndarray):
 def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: float, n_types: int,
                  n_solvents: int | None, T0: float = 300.0, dt: float = DT,
                  solvent_density: float = 33.0, compound_size: int = COMPOUND_SIZE,
-                 forcefield: np.ndarray | None = None) -> LimaSystem:
-    """Build one seeded system.  n_solvents=None keeps every free lattice site."""
+                 forcefield: np.ndarray | None = None, confine: float | None = None,
+                 central_first: bool = False, sphere_margin: float = 0.0) -> LimaSystem:
+    """Build one seeded system.  n_solvents=None keeps every free lattice site.
+
+    confine: largest distance of a compound's particles from its first walk bead
+    (default 0.38 L, less a margin in boxes under 5 nm).  central_first: number
+    each compound from the bead nearest its centroid, so that the particle every
+    other one is re-imaged against (Engine.cu:993-996) sits in the middle.
+    sphere_margin: added to each compound's confining sphere (Bodies.cuh:407-423),
+    which widens its neighbour-list radii for chains that swell during the run."""
     rng = np.random.default_rng(seed)
     ff = make_forcefield(rng, n_types) if forcefield is None else forcefield.astype(np.float32)
     L = float(box_len)
     n_beads_chain = compounds_per_chain * compound_size
     # every compound stays well inside half a box of its particle 0, against which
     # its positions are re-imaged: 0.38 L, less a margin in boxes under 5 nm
-    confine = 0.38 * L - max(0.0, 0.6 - 0.12 * L)
+    if confine is None:
+        confine = 0.38 * L - max(0.0, 0.6 - 0.12 * L)
     walk = (_walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size) if n_chains > 0
             else np.zeros((0, 3)))
     n_chain = len(walk)
+    # walk order -> particle index: identity, or each compound's central bead first
+    w2g = np.arange(n_chain)
+    if central_first:
+        for c0 in range(0, n_chain, compound_size):
+            seg = walk[c0:c0 + compound_size]
+            k = int(np.argmin(((seg - seg.mean(axis=0)) ** 2).sum(axis=1)))
+            order = [k] + [i for i in range(len(seg)) if i != k]
+            w2g[c0 + np.array(order)] = c0 + np.arange(len(seg))
+    walk_g = np.zeros_like(walk)
+    walk_g[w2g] = walk
     types = rng.integers(1, n_types + 1, size=n_chain).astype(np.uint8)
     masses = ff[types, 0].astype(np.float64)
 
@@ -213,13 +232,14 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
     coff = np.arange(nc + 1, dtype=np.int32) * compound_size
     cpos = np.zeros((n_chain, 3), dtype=np.float32)
     for c in range(nc):
-        seg = walk[coff[c]:coff[c + 1]]
+        seg = walk_g[coff[c]:coff[c + 1]]
         p0 = np.mod(seg[0], L)
         cpos[coff[c]:coff[c + 1]] = (p0 + (seg - seg[0])).astype(np.float32)
     key = np.zeros(nc, dtype=np.int32)
     sphere = np.zeros(nc, dtype=np.float32)
     for c in range(nc):
         key[c], sphere[c] = _compound_sphere(cpos[coff[c]:coff[c + 1]])
+        sphere[c] += np.float32(sphere_margin)
 
     # ---- topology: compound terms, bridges, exclusions ----
     comp_of = np.arange(n_chain) // compound_size
@@ -254,11 +274,11 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
         s = ch * n_beads_chain
         e = s + n_beads_chain
         for i in range(s, e - 1):
-            add_term("b", (i, i + 1))
+            add_term("b", tuple(int(w2g[k]) for k in (i, i + 1)))
         for i in range(s, e - 2):
-            add_term("a", (i, i + 1, i + 2))
+            add_term("a", tuple(int(w2g[k]) for k in (i, i + 1, i + 2)))
         for i in range(s, e - 3):
-            add_term("d", (i, i + 1, i + 2, i + 3))
+            add_term("d", tuple(int(w2g[k]) for k in (i, i + 1, i + 2, i + 3)))
 
     def pack(lists, kind):
         off = np.zeros(len(lists) + 1, dtype=np.int32)
@@ -368,20 +388,33 @@ def large() -> LimaSystem:
     return build_system(SEED_BASE + 2000, 10, 40, 18.0, 16, None, T0=300.0)
 
 
+REPLICA_COMPOUND = 64      # MAX_COMPOUND_PARTICLES (Constants.cuh:58)
+
+
 def replicas(n: int = 256, seed_config: int = 3) -> list[LimaSystem]:
-    """Uneven replicas: chain 128-1024 (1-8 compounds), solvent 500-6000, box from density 33 (3.6-6 nm)."""
+    """Uneven replicas: chain 128-1024 (2-16 compounds of 64), solvent 500-6000, each with its own
+    thermostat target U(280, 320) K.
+
+    The chains of this force field stretch (the 1-5 LJ repulsion at 0.15 nm bonds favours trans
+    torsions), and the reference re-images every compound against its particle 0 and lists
+    neighbours with spheres fixed at build time.  So that a replica stays valid for ~3,000 steps,
+    compounds hold 64 particles, are numbered from their central bead and confined to 1.3 nm of
+    their first bead, their spheres carry a 1.5 nm margin, and the box edge scaled to density
+    33/nm^3 is kept within [5.5, 6] nm."""
     out = []
     for r in range(n):
         seed = SEED_BASE + 1000 * seed_config + r
         rng = np.random.default_rng(seed ^ 0x5EED)
-        ncomp = int(rng.integers(1, 9))
+        ncomp = int(rng.integers(1, 9))                 # chain of 128 * ncomp particles
         nsol = int(rng.integers(500, 6001))
         T0 = float(rng.uniform(280.0, 320.0))
-        L = ((ncomp * COMPOUND_SIZE + nsol) / 33.0) ** (1.0 / 3.0)
-        L = float(min(6.0, max(3.6, L)))
-        cap = int(0.8 * 33.0 * L ** 3) - ncomp * COMPOUND_SIZE * 3
+        nc = ncomp * COMPOUND_SIZE // REPLICA_COMPOUND
+        L = ((nc * REPLICA_COMPOUND + nsol) / 33.0) ** (1.0 / 3.0)
+        L = float(min(6.0, max(5.5, L)))
+        cap = int(0.8 * 33.0 * L ** 3) - nc * REPLICA_COMPOUND * 3
         nsol = max(100, min(nsol, cap))
-        out.append(build_system(seed, 1, ncomp, L, 16, nsol, T0=T0))
+        out.append(build_system(seed, 1, nc, L, 16, nsol, T0=T0, compound_size=REPLICA_COMPOUND, confine=1.3,
+                                central_first=True, sphere_margin=1.5))
     return out
 
 
