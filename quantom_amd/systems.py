@@ -422,3 +422,38 @@ def adjoint_replicas(n: int = 64) -> list[LimaSystem]:
     """64 replicas of 1 chain of 512 + 1,488 solvent in a 4.2 nm box, 16 types (shared force field)."""
     ff = make_forcefield(np.random.default_rng(SEED_BASE + 4000 - 1), 16)
     return [build_system(SEED_BASE + 4000 + r, 1, 4, 4.2, 16, 1488, T0=300.0, forcefield=ff) for r in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# BoxBuilder's initial velocities (BoxBuilder.cu:248-276, BoxBuilder.cuh:56-102)
+# ---------------------------------------------------------------------------
+BOXBUILDER_T = 313.0                       # BoxBuilder::T [K] (BoxBuilder.cuh:63)
+BOXBUILDER_R = 8.3144                      # BoxBuilder::R (BoxBuilder.cuh:64)
+SOLVENT_MASS = float(np.float32(18.01528)) * 1e-3   # constexpr double from a float literal (Simulation.cuh:30)
+
+
+def reference_initial_velocities(system: LimaSystem, seed: int = 290128309) -> LimaSystem:
+    """The velocities BoxBuilder gives a fresh box, as previous positions.
+
+    Solvents (createSolvent, BoxBuilder.cu:273-276): direction = norm of three
+    draws of random() = (r % 10000) / 10000.f * 2 - 1 (uniform in the cube, so
+    the direction is not isotropic), speed v_rms = sqrt(3 R T / M) with T = 313 K
+    and M = SOLVENT_MASS; prev = pos - vel * dt, the FP64 products narrowed to
+    FP32 as Float3 * double does (QuantomTypes.cuh:39).  Compounds
+    (integrateCompound, BoxBuilder.cu:248-269): zero velocity, prev = pos.
+    The draws come from numpy's generator seeded with BoxBuilder's srand seed
+    (BoxBuilder.cuh:10); they are not the C library's rand() sequence."""
+    out = system.copy()
+    rng = np.random.default_rng(seed)
+    n = system.n_solvents
+    r = rng.integers(0, 10000, size=(n, 3))
+    comp = (r.astype(np.float32) / np.float32(10000.0) * np.float32(2) - np.float32(1)).astype(np.float32)
+    norm = np.sqrt(((comp * comp).sum(axis=1, dtype=np.float32))).astype(np.float32)
+    norm[norm == 0] = np.float32(1)
+    unit = (comp / norm[:, None]).astype(np.float32)
+    v_rms = math.sqrt(3 * BOXBUILDER_R * BOXBUILDER_T / SOLVENT_MASS)
+    vel = (unit.astype(np.float64) * v_rms).astype(np.float32)
+    step = (vel.astype(np.float64) * system.dt).astype(np.float32)
+    out.solvent_prev_positions = (system.solvent_positions.astype(np.float32) - step).astype(np.float32)
+    out.chain_prev_positions = system.chain_positions.astype(np.float32).copy()
+    return out

@@ -155,3 +155,24 @@ def test_reference_thermostat_matches_oracle(small_system):
     print("max |dx|", d.max(), "lambda", lam_gpu[40], lam_ora[40], lam_gpu[80], lam_ora[80])
     assert d.max() <= 1e-3
     eng.close()
+
+
+def test_boxbuilder_initial_velocities(small_system):
+    """BoxBuilder's start (BoxBuilder.cu:248-276): solvents at v_rms = sqrt(3 R T / M) with
+    T = 313 K, R = 8.3144 and the solvent mass; compounds at rest.  The box temperature the
+    engine's one-step kinetic energy then reads is N_solvent / N of that, in closed form."""
+    import math
+
+    from oracle.oracle import Oracle
+    from quantom_amd import systems
+    s = systems.reference_initial_velocities(small_system)
+    v_rms = math.sqrt(3 * 8.3144 * 313.0 / (float(np.float32(18.01528)) * 1e-3))
+    v = np.linalg.norm((s.solvent_positions.astype(np.float64) - s.solvent_prev_positions) / s.dt, axis=1)
+    assert np.allclose(v, v_rms, rtol=1e-3)                 # FP32 positions: |dx| = 1.3e-3 nm, ulp 5e-7
+    assert np.array_equal(s.chain_prev_positions, s.chain_positions)
+    assert np.array_equal(s.solvent_positions, small_system.solvent_positions)
+    m_sol = float(s.forcefield[0, 0])
+    t_expect = s.n_solvents * m_sol * v_rms ** 2 / (3 * GAS_R * (s.n_chain + s.n_solvents))
+    o = Oracle(s, RunParams(nlist_interval=10, thermostat=THERMOSTAT_REFERENCE), precision=64)
+    t = o.energies()["temperature"]
+    assert abs(t - t_expect) / t_expect < 2e-3, (t, t_expect)
