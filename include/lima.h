@@ -198,6 +198,22 @@ int lima_get_energies(lima_engine* engine, lima_energies* out);
    step's displacement (what the thermostat uses).  out: [ceil(n_steps/every)][n_systems].
    One host synchronisation at the end. */
 int lima_step_observed(lima_engine* engine, int32_t n_steps, int32_t every, lima_energies* out);
+/* lima_step with the per-step logs the reference keeps for one system
+   (Engine::offloadLoggingData / offloadTrainData, Engine.cu:288-310; the
+   compoundKernel's data log, Engine.cu:1019-1046), copied out every 100 steps.
+   logdata:   [n_steps][10] floats (logdata.bin): for particle 0 of the system's
+              first compound, the speed |x' - x|/dt, the kinetic energy
+              calcKineticEnergy gives for those one-step-apart positions,
+              thread 0's potE slot (its first dihedral's energy, which the
+              reference assigns, else its first bond and angle), |F|; then six 0.
+   traindata: [n_steps][n_compounds][slots_per_compound][3][3] floats
+              (sim_traindata.bin, data_GAN): position after the step (-1 in slots
+              past the compound), the compound<-solvent LJ force (Float3(-1) with
+              solvents off), and a third Float3 written 0.
+   Either output may be NULL (not both); slots_per_compound is the reference
+   build's MAX_COMPOUND_PARTICLES and must cover the system's largest compound. */
+int lima_step_logged(lima_engine* engine, int32_t n_steps, int32_t system, int32_t slots_per_compound,
+                     float* logdata, float* traindata);
 
 /* neighbour lists (NeighborList, Bodies.cuh:203-292) as global ids within a system.
    counts: [n_compounds] / [n_solvents]; ids: stride = the matching capacity. */

@@ -238,6 +238,7 @@ template <typename T> struct Oracle {
     // ---- state ----
     std::vector<std::vector<int>> cc, cs, sc, ss;   // NeighborList contents, in list order
     std::vector<V3<T>> fbridge, cforce, sforce;
+    std::vector<V3<T>> csforce;   // per chain particle: computeSolventToCompoundLJForces' result, -1s when not run
     T lambda = T(1);
     long step = 0;
     int err = LIMA_OK;
@@ -371,6 +372,7 @@ template <typename T> struct Oracle {
                     V3<T> f;
                     for (size_t k = 0; k < stage.size(); k++) f += lj_force(pos[p], stage[k], sgm, ep, pes, pvi);
                     force[p] += f;
+                    csforce[base + p] = f;
                 }
             }
             e_cs += es;
@@ -708,6 +710,7 @@ template <typename T> int build(Oracle<T>& o, const lima_system* s, const lima_r
     o.cc.assign(o.nc, {}); o.cs.assign(o.nc, {}); o.sc.assign(o.ns, {}); o.ss.assign(o.ns, {});
     o.fbridge.assign(o.nchain, V3<T>());
     o.cforce.assign(o.nchain, V3<T>());
+    o.csforce.assign(o.nchain, V3<T>(T(-1), T(-1), T(-1)));
     o.sforce.assign(o.ns, V3<T>());
     o.update_neighbors();                                                       // Engine.cu:35
     return o.err;
@@ -771,6 +774,18 @@ int oracle_compute_forces(void* h, double* cf, double* sf) {
 int oracle_get_forces(void* h, double* cf, double* sf) {
     DISPATCH(h, { put3(o.cforce, cf); put3(o.sforce, sf); });
     return 0;
+}
+
+// computeSolventToCompoundLJForces of the last force pass, per chain particle
+int oracle_get_cs_forces(void* h, double* out) {
+    int st = LIMA_OK;
+    DISPATCH(h, {
+        for (int i = 0; i < o.nchain; i++) {
+            out[3 * i] = (double)o.csforce[i].x; out[3 * i + 1] = (double)o.csforce[i].y; out[3 * i + 2] = (double)o.csforce[i].z;
+        }
+        st = o.err;
+    });
+    return st;
 }
 
 int oracle_rebuild_neighbors(void* h) {

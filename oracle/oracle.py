@@ -40,6 +40,7 @@ def lib(fast: bool = False) -> C.CDLL:
         L.oracle_get_lambda.restype = C.c_double
         L.oracle_get_lambda.argtypes = [C.c_void_p]
         L.oracle_set_lambda.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_get_cs_forces.argtypes = [C.c_void_p, dp]
         L.oracle_get_ref_temperature.restype = C.c_double
         L.oracle_get_ref_temperature.argtypes = [C.c_void_p]
         L.oracle_get_step.restype = C.c_long
@@ -118,6 +119,12 @@ class Oracle:
         sf = np.zeros((self.ns, 3))
         self.L.oracle_get_forces(self.h, _dp(cf), _dp(sf))
         return cf, sf
+
+    def cs_forces(self):
+        """computeSolventToCompoundLJForces of the last force pass per chain particle (-1s when not run)."""
+        out = np.zeros((self.nchain, 3))
+        self.L.oracle_get_cs_forces(self.h, _dp(out))
+        return out
 
     def rebuild_neighbors(self) -> int:
         return self.L.oracle_rebuild_neighbors(self.h)

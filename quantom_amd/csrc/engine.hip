@@ -151,6 +151,10 @@ struct lima_engine {
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
     DBuf<long long> step_count;   // per replica: steps done (the reference cadence thermostat reads it)
+    // per-step logs (lima_step_logged): replica, slots per compound, ring length and rings
+    int log_rep = -1, log_slots = 0, log_ring = 1;
+    bool log_train = false;
+    DBuf<float> log_data, train_data;
     DBuf<uint32_t> err;
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
     DBuf<float4> ss_part, ss_img;
@@ -241,6 +245,12 @@ struct lima_engine {
         a.ke_comp = ke_comp.p;
         a.ke_stile = ke_stile.p;
         a.ke_ref = params.thermostat == LIMA_THERMOSTAT_REFERENCE ? 1 : 0;
+        a.log_rep = log_rep;
+        a.log_slots = log_slots;
+        a.log_ring = log_ring;
+        a.step_count = step_count.p;
+        a.log_data = log_data.p;
+        a.train_data = log_rep >= 0 && log_train ? train_data.p : nullptr;
         a.energy = energy.p;
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
@@ -349,7 +359,7 @@ struct lima_engine {
         // s2: bridge carry-in forces, then the listed solvent-solvent LJ
         if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
         HIP_TRY(hipEventRecord(ev_bridge, s2));
-        if (solv) solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles), 256, 0, s2>>>(a, n_stiles);
+        if (solv) solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles * kListedItems), 64, 0, s2>>>(a, n_stiles * kListedItems);
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s1: compound listed LJ, then bonded + intra + integration of the compounds
         if (NC > 0) {
@@ -1608,6 +1618,66 @@ int lima_get_energies(lima_engine* E, lima_energies* out) {
     });
 }
 
+// Steps with the reference's per-step logs of one system: Engine::offloadLoggingData /
+// offloadTrainData (Engine.cu:288-310) move device rings to the host every 100 steps
+// (STEPS_PER_TRAINDATATRANSFER, Constants.cuh:99); here the rings are copied into the
+// caller's arrays at the same cadence.
+constexpr int kLogRing = 100;
+
+int lima_step_logged(lima_engine* E, int32_t n, int32_t system, int32_t slots, float* logdata, float* traindata) {
+    return guarded(E, [&] {
+        if (n < 0 || system < 0 || system >= E->R) throw LimaError(LIMA_E_ARG, "n_steps < 0 or system out of range");
+        if (n > 0 && !logdata && !traindata) throw LimaError(LIMA_E_ARG, "no output");
+        const ReplicaInfo& ri = E->reps_h[system];
+        const int nc = ri.comp_end - ri.comp_begin;
+        int maxp = 0;
+        for (int c = ri.comp_begin; c < ri.comp_end; c++) maxp = std::max(maxp, E->comp_off_h[c + 1] - E->comp_off_h[c]);
+        if (traindata && (slots < std::max(maxp, 1) || slots > kCompoundBlock))
+            throw LimaError(LIMA_E_ARG, "slots_per_compound must cover the largest compound and be at most 128");
+        if (nc == 0) throw LimaError(LIMA_E_ARG, "the system has no compounds to log");
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        const size_t per_step = (size_t)nc * std::max(slots, 1) * 9;
+        if (E->log_data.n < (size_t)kLogRing * 10) E->log_data.alloc((size_t)kLogRing * 10);
+        if (traindata && E->train_data.n < kLogRing * per_step) E->train_data.alloc(kLogRing * per_step);
+        // the rings enter the captured step through its arguments: recapture with them, and again without
+        auto reset_graph = [&] {
+            if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+        };
+        struct Off {   // logging is off again however the call ends
+            lima_engine* E;
+            ~Off() {
+                E->log_rep = -1;
+                E->log_train = false;
+                if (E->graph_exec) { (void)hipGraphExecDestroy(E->graph_exec); E->graph_exec = nullptr; }
+            }
+        } off{E};
+        E->log_rep = system;
+        E->log_train = traindata != nullptr;
+        E->log_slots = traindata ? slots : 0;
+        E->log_ring = kLogRing;
+        reset_graph();
+        std::vector<float> hl((size_t)kLogRing * 10), ht(traindata ? kLogRing * per_step : 0);
+        int st = LIMA_OK;
+        for (int done = 0; done < n && st == LIMA_OK;) {
+            const int chunk = std::min(kLogRing, n - done);
+            const long long step0 = E->step;
+            E->enqueue_steps(chunk);
+            HIP_TRY(hipStreamSynchronize(E->stream));
+            st = E->check_errors();
+            HIP_TRY(hipMemcpy(hl.data(), E->log_data.p, hl.size() * 4, hipMemcpyDeviceToHost));
+            if (traindata) HIP_TRY(hipMemcpy(ht.data(), E->train_data.p, ht.size() * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < chunk; i++) {
+                const size_t ring = (size_t)((step0 + i) % kLogRing);
+                if (logdata) std::memcpy(logdata + (size_t)(done + i) * 10, hl.data() + ring * 10, 40);
+                if (traindata) std::memcpy(traindata + (size_t)(done + i) * per_step, ht.data() + ring * per_step, per_step * 4);
+            }
+            done += chunk;
+        }
+        return st;
+    });
+}
+
 int lima_step_observed(lima_engine* E, int32_t n, int32_t every, lima_energies* out) {
     return guarded(E, [&] {
         if (n < 0 || every <= 0) throw LimaError(LIMA_E_ARG, "n_steps < 0 or every <= 0");
@@ -1787,7 +1857,9 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
             });
         }
         if (E->params.enable_solvents && E->NS > 0)
-            out8[6] = timeit([&] { solvent_listed_kernel<false><<<xcd_grid(E->n_stiles), 256, 0, E->stream>>>(a, E->n_stiles); });
+            out8[6] = timeit([&] {
+                solvent_listed_kernel<false><<<xcd_grid(E->n_stiles * kListedItems), 64, 0, E->stream>>>(a, E->n_stiles * kListedItems);
+            });
         if (E->NC > 0) {
             const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
             out6[1] = timeit([&] {

@@ -85,6 +85,13 @@ struct StepArgs {
     double* ke_comp;              // [n_comp]
     double* ke_stile;             // [n_stiles]
     int ke_ref;                   // 1: the thermostat sums read the reference's two-step kinetic energy
+    // per-step logs of one replica (the compoundKernel's data log, Engine.cu:1019-1046)
+    int log_rep;                  // logged replica, -1: none
+    int log_slots;                // particle slots per compound (the reference's MAX_COMPOUND_PARTICLES)
+    int log_ring;                 // steps the device rings hold; step k writes slot k % log_ring
+    const long long* step_count;  // [R] steps done before this one
+    float* log_data;              // [log_ring][10]
+    float* train_data;            // [log_ring][compounds of log_rep][log_slots][3 x Float3], or null
     // energies: [R*kEnergyTerms]
     double* energy;
     uint32_t* err;                // [4]
@@ -537,12 +544,24 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
     if (FIXED) { fx_i += to_fixed(lx); fy_i += to_fixed(ly); fz_i += to_fixed(lz); }
     else force = f3_add(force, f3(lx, ly, lz));
     // listed partials: neighbour compounds (list order), then solvent chunks
+    const bool logged = ADVANCE && a.log_rep == r;
+    F3 f_sol = f3(-1.f, -1.f, -1.f);   // computeSolventToCompoundLJForces' result, Float3(-1) when unset (Engine.cu:919)
     if (active) {
         const int s0 = a.cw_off[c], s1 = a.cw_off[c + 1];
         for (int s = s0; s < s1; s++) {
             const size_t slot = (size_t)s * kCompoundBlock + p;
             if (FIXED) { I3 v = a.partC_fx[slot]; fx_i += v.x; fy_i += v.y; fz_i += v.z; }
             else { float4 v = a.partC[slot]; force.x += v.x; force.y += v.y; force.z += v.z; }
+        }
+        if (logged && a.enable_solvents) {
+            long long sx_i = 0, sy_i = 0, sz_i = 0;
+            float gx = 0.f, gy = 0.f, gz = 0.f;
+            for (int s = s0 + a.cc_cnt[c]; s < s1; s++) {
+                const size_t slot = (size_t)s * kCompoundBlock + p;
+                if (FIXED) { I3 v = a.partC_fx[slot]; sx_i += v.x; sy_i += v.y; sz_i += v.z; }
+                else { float4 v = a.partC[slot]; gx += v.x; gy += v.y; gz += v.z; }
+            }
+            f_sol = FIXED ? f3(from_fixed(sx_i), from_fixed(sy_i), from_fixed(sz_i)) : f3(gx, gy, gz);
         }
     }
     if (FIXED) force = f3(from_fixed(fx_i), from_fixed(fy_i), from_fixed(fz_i));
@@ -567,6 +586,31 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
         float zx = pbc1(z.x, L), zy = pbc1(z.y, L), zz = pbc1(z.z, L);
         hyperpos(zx, zy, zz, px, py, pz, L, half);
         if (active) a.cpos_next[gi] = make_float4(px, py, pz, me.w);
+        if (logged) {
+            const long long k = a.step_count[r];
+            const int ring = (int)(k % a.log_ring);
+            const int cl = c - rp.comp_begin;
+            if (a.train_data && p < a.log_slots) {
+                // data_GAN (Engine.cu:1036-1043): position, solvent LJ force, and a third Float3 left zero
+                float* t = a.train_data + (((size_t)ring * (rp.comp_end - rp.comp_begin) + cl) * a.log_slots + p) * 9;
+                const bool in = p < n;
+                t[0] = in ? px : -1.f; t[1] = in ? py : -1.f; t[2] = in ? pz : -1.f;
+                t[3] = f_sol.x; t[4] = f_sol.y; t[5] = f_sol.z;
+                t[6] = 0.f; t[7] = 0.f; t[8] = 0.f;
+            }
+            if (cl == 0 && p == 0) {
+                // outdata of compound 0, particle 0 (Engine.cu:1019-1028): speed, kinetic energy of the
+                // one-step-apart positions, thread 0's potE slot (the dihedral assigns it), |F|
+                float bx = qx, by = qy, bz = qz;
+                hyperpos(px, py, pz, bx, by, bz, L, half);
+                float* o = a.log_data + (size_t)ring * 10;
+                o[0] = (float)((double)exact_len(fsub(px, bx), fsub(py, by), fsub(pz, bz)) / rp.dt);
+                o[1] = ref_kinetic(px, py, pz, bx, by, bz, m, rp.dt, L, half);
+                o[2] = (float)((a.enable_dihedrals && nd > 0) ? ed : eb + ea);
+                o[3] = exact_len(force.x, force.y, force.z);
+                for (int j = 4; j < 10; j++) o[j] = 0.f;
+            }
+        }
         // the logged position of this step against the one two steps back (Engine.cu:333-337)
         kt = !active ? 0.0 : a.ke_ref ? (double)ref_kinetic(px, py, pz, pv.x, pv.y, pv.z, m, rp.dt, L, half) : ke;
     } else if (active) {
@@ -607,13 +651,13 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
 // image are written for the integrator.  One thread per solvent; it runs
 // concurrently with the all-pairs kernel.
 //
-// The lists are rows of ss_ld ids (a multiple of kListChunk).  A wavefront reads
+// One wavefront per block, 64 solvents: work item = (solvent tile, quarter).  The
+// lists are rows of ss_ld ids (a multiple of kListChunk).  The wavefront reads
 // the rows of its 64 solvents kListChunk entries at a time, four rows per load
-// instruction (one 64-byte segment each), and transposes them through its own
-// LDS stage; each lane then walks its row from LDS.  Every list byte is read
-// from HBM once, in whole segments, which leaves L2 to the position gathers.
-// Tiles are taken in XCD-contiguous order (xcd_item), so the tiles of one
-// replica share one L2.
+// instruction (one 64-byte segment each), and transposes them through LDS; each
+// lane then walks its row from LDS.  Every list byte is read from HBM once, in
+// whole segments, which leaves L2 to the position gathers.  Items are taken in
+// XCD-contiguous order (xcd_item), so the tiles of one replica share one L2.
 // ---------------------------------------------------------------------------
 constexpr int kListChunk = 16;                 // list entries per lane per stage
 constexpr int kListStride = kListChunk + 4;    // LDS row stride in ints (16-byte aligned, conflict-free b128 reads)
@@ -628,18 +672,22 @@ LIMA_DEV int xcd_item(int b, int nblocks) {
 }
 inline int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
 
+constexpr int kListedItems = 4;   // wavefront items per 256-solvent tile
+
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a, int n_tiles) {
-    // the per-wave list stages; the replay's particle-0 stage aliases them (used before the lists)
-    __shared__ __align__(16) int stage[4 * 64 * kListStride];
+__global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_items) {
+    // the wavefront's list stage; the replay's particle-0 stage aliases it (used before the lists)
+    __shared__ __align__(16) int stage[64 * kListStride];
     static_assert(sizeof(stage) >= kReplayStage * sizeof(float4), "replay stage must fit the list stage");
-    const int t = xcd_item(blockIdx.x, gridDim.x);
-    if (t >= n_tiles) return;
-    const int4 tile = a.stiles[t];
+    const int item = xcd_item(blockIdx.x, gridDim.x);
+    if (item >= n_items) return;
+    const int4 tile = a.stiles[item / kListedItems];
+    const int wrow0 = tile.y + (item % kListedItems) * 64;   // the wavefront's first solvent
+    if (wrow0 >= tile.z) return;
     const int r = tile.x;
     const ReplicaInfo rp = a.reps[r];
     const float L = rp.L, half = rp.half;
-    const int s = tile.y + threadIdx.x;
+    const int s = wrow0 + threadIdx.x;
     const bool active = s < tile.z;
     const int si = active ? s : tile.y;
     const float4 me = a.spos[si];
@@ -651,9 +699,8 @@ __global__ __launch_bounds__(256) void solvent_listed_kernel(StepArgs a, int n_t
     float lx = 0.f, ly = 0.f, lz = 0.f;
     double en = 0, vir = 0;
     const int cnt = active ? a.ss_cnt[si] : 0;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int* st = stage + wid * 64 * kListStride;
-    const int wrow0 = tile.y + wid * 64;        // the wave's first solvent (rows beyond tile.z have count 0)
+    const int lane = threadIdx.x;
+    int* st = stage;                            // rows beyond tile.z have count 0
     int wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
     const int sub = lane >> 4, col = lane & (kListChunk - 1);

@@ -30,7 +30,7 @@ EXPORTS = (
     "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
     "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_get_config",
     "lima_autotune", "lima_get_parameters", "lima_set_parameters", "lima_adjoint_run", "lima_record_frames",
-    "lima_step_observed",
+    "lima_step_observed", "lima_step_logged",
 )
 
 
@@ -78,6 +78,7 @@ def load_library() -> C.CDLL:
     L.lima_adjoint_run.argtypes = [vp, C.POINTER(CAdjointSpec), dp, dp]
     L.lima_record_frames.argtypes = [vp, C.c_int32, fp, fp, dp]
     L.lima_step_observed.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(CEnergies)]
+    L.lima_step_logged.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, fp, fp]
     _lib = L
     return L
 
@@ -207,6 +208,17 @@ class Engine:
         self._check(self.L.lima_step_observed(self.h, n, every, arr))
         self.steps += n
         return [[arr[k * R + r].as_dict() for r in range(R)] for k in range(nrec)]
+
+    def step_logged(self, n: int, system: int = 0, slots_per_compound: int = 64, train: bool = True):
+        """n steps with the reference's per-step logs of one system (logdata.bin, sim_traindata.bin).
+        Returns (logdata [n, 10], traindata [n, n_compounds, slots, 3, 3] or None)."""
+        s = self.simulation.systems[system]
+        log = np.zeros((max(n, 1), 10), np.float32)
+        tr = np.zeros((max(n, 1), s.n_compounds, slots_per_compound, 3, 3), np.float32) if train else None
+        self._check(self.L.lima_step_logged(self.h, n, system, slots_per_compound, _f(log),
+                                            _f(tr) if train else None))
+        self.steps += n
+        return log[:n], (tr[:n] if train else None)
 
     # ---- neighbour lists (global ids within the concatenation) ----
     def neighbor_counts(self):

@@ -13,7 +13,8 @@ Environment.cpp:256-266):
 
 Here the series comes from ``Engine.step_observed`` (lima_step_observed): one
 record per observed step, taken in the step itself instead of from a stored
-trajectory.
+trajectory.  ``logdata.bin`` and ``sim_traindata.bin`` (Environment.cpp:130-141)
+come from ``Engine.step_logged`` (lima_step_logged).
 """
 from __future__ import annotations
 
@@ -44,4 +45,19 @@ def write_reference_dumps(records: list, engine, out_dir: str, system: int = 0, 
     paths = {"energy": os.path.join(out_dir, "energy.bin"), "temperature": os.path.join(out_dir, "temperature.bin")}
     energy_series(records, system, n, lj).astype("<f4").tofile(paths["energy"])
     temperature_series(records, system).astype("<f4").tofile(paths["temperature"])
+    return paths
+
+
+def write_reference_logs(logdata: np.ndarray, traindata: np.ndarray | None, out_dir: str,
+                         critical_error: bool = False) -> dict:
+    """Writes logdata.bin (10 float32 per step) and sim_traindata.bin (per step, compound and
+    particle slot: 3 Float3 = 9 float32), as Environment::postRunEvents does
+    (Environment.cpp:130-141); the training data is skipped after a critical error, as there.
+    The arrays are Engine.step_logged's."""
+    os.makedirs(out_dir, exist_ok=True)
+    paths = {"logdata": os.path.join(out_dir, "logdata.bin")}
+    np.ascontiguousarray(logdata, dtype="<f4").tofile(paths["logdata"])
+    if traindata is not None and not critical_error:
+        paths["traindata"] = os.path.join(out_dir, "sim_traindata.bin")
+        np.ascontiguousarray(traindata, dtype="<f4").tofile(paths["traindata"])
     return paths
