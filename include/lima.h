@@ -232,8 +232,9 @@ int lima_get_timings(lima_engine* engine, double* out5);
 /* Average device time in microseconds of each kernel of the step, each launched
    `reps` times alone on the engine's stream between HIP events, on the current
    state: [0] solvent<-chain all-pairs, [1] compound listed LJ, [2] compound
-   bonded+intra+integrate (forces only), [3] solvent integrate (forces only),
-   [4] bridge, [5] full neighbour-list rebuild, [6] solvent listed LJ, [7] 0. */
+   sums + integrate (forces only), [3] solvent integrate (forces only),
+   [4] bridge, [5] full neighbour-list rebuild, [6] solvent listed LJ,
+   [7] compound local forces (bridge carry-in, bonded, intra-compound LJ). */
 int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
 /* per-kernel launch shape chosen for this engine (see lima_autotune) */
 int lima_get_config(lima_engine* engine, int32_t* out8);

@@ -145,8 +145,8 @@ struct lima_engine {
     DBuf<int> cc_cnt, cc_ids, cs_cnt, cs_ids, ss_cnt, ss_ids, seg_cnt;
     int max_segs = 1;
     DBuf<int> cw_off, cw_total;
-    DBuf<float4> partC, partS;
-    DBuf<I3> partC_fx, partS_fx;
+    DBuf<float4> partC, partS, cloc;
+    DBuf<I3> partC_fx, partS_fx, cloc_fx;
     DBuf<SCWork> scwork;
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
@@ -161,11 +161,12 @@ struct lima_engine {
     DBuf<float> comp_cs_bound;
     AdjBufs adjb;
 
-    // the step is a fork/join DAG over three streams (all at default priority):
+    // the step is a fork/join DAG over four streams (all at default priority):
     // stream: solvent<-chain all-pairs -> solvent integrate -> thermostat [-> rebuild]
-    // s1:     compound listed LJ -> compound integrate
-    // s2:     bridge, solvent listed LJ
-    hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr;
+    // s1:     compound listed LJ -> (after s3) compound sums + integrate
+    // s2:     solvent listed LJ
+    // s3:     bridge -> compound local forces
+    hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
@@ -182,7 +183,7 @@ struct lima_engine {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c})
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t s : {stream, s1, s2})
+        for (hipStream_t s : {stream, s1, s2, s3})
             if (s) (void)hipStreamDestroy(s);
     }
 
@@ -190,6 +191,7 @@ struct lima_engine {
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
@@ -238,6 +240,7 @@ struct lima_engine {
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
+        a.cloc = cloc.p; a.cloc_fx = cloc_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
         a.scwork = scwork.p; a.n_scwork = (int)scwork_h.size();
         a.chunk_cap = chunk_cap;
@@ -349,19 +352,22 @@ struct lima_engine {
         scwork.upload(scwork_h);
     }
 
-    // one force pass (and integration when ADVANCE) as a fork/join over three streams
+    // one force pass (and integration when ADVANCE) as a fork/join over four streams
     template <bool ADVANCE, bool FIXED, bool ENERGY> void launch_forces(int parity) {
         StepArgs a = step_args(parity);
         const bool solv = params.enable_solvents && NS > 0;
         HIP_TRY(hipEventRecord(ev_fork, stream));
         HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
         HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
-        // s2: bridge carry-in forces, then the listed solvent-solvent LJ
-        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s2>>>(a);
-        HIP_TRY(hipEventRecord(ev_bridge, s2));
+        HIP_TRY(hipStreamWaitEvent(s3, ev_fork, 0));
+        // s3: bridge carry-in forces, then the compounds' local forces (no list needed)
+        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s3>>>(a);
+        if (NC > 0) compound_local_kernel<FIXED, ENERGY><<<NC, kCompoundBlock, 0, s3>>>(a);
+        HIP_TRY(hipEventRecord(ev_bridge, s3));
+        // s2: the listed solvent-solvent LJ
         if (solv) solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles * kListedItems), 64, 0, s2>>>(a, n_stiles * kListedItems);
         HIP_TRY(hipEventRecord(ev_ss, s2));
-        // s1: compound listed LJ, then bonded + intra + integration of the compounds
+        // s1: compound listed LJ, then (after the local forces) the sums and integration of the compounds
         if (NC > 0) {
             const int grid = cfg.listed_grid > 0 ? cfg.listed_grid : 4096;
             compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, s1>>>(a);
@@ -785,7 +791,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->stiles.upload(stl);
     E->mass.upload(mass);
     E->tab_sc.upload(tsc); E->tab_cs.upload(tcs); E->tab_inter.upload(tinter); E->tab_intra.upload(tintra); E->tab_ss.upload(tss);
-    const size_t NCx = std::max(E->NC, 1), NSx = std::max(E->NS, 1);
+    const size_t NCx = std::max(E->NC, 1), NSx = std::max(E->NS, 1), NCHx = std::max(E->NCH, 1);
     E->cc_cnt.alloc(NCx); E->cc_cnt.zero();
     E->cc_ids.alloc(NCx * P.max_compound_neighbors);
     E->cs_cnt.alloc(NCx); E->cs_cnt.zero();
@@ -804,9 +810,13 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     if (P.accumulation == LIMA_ACCUM_FIXED64) {
         E->partC_fx.alloc(E->max_slots * kCompoundBlock);
         E->partC.alloc(1);
+        E->cloc_fx.alloc(NCHx);
+        E->cloc.alloc(1);
     } else {
         E->partC.alloc(E->max_slots * kCompoundBlock);
         E->partC_fx.alloc(1);
+        E->cloc.alloc(NCHx);
+        E->cloc_fx.alloc(1);
     }
     std::vector<float> ones(nsys, 1.0f);
     E->lambda.upload(ones);
@@ -1870,6 +1880,10 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
                 if (fx) compound_integrate_kernel<false, true, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
                 else compound_integrate_kernel<false, false, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
             });
+            out8[7] = timeit([&] {
+                if (fx) compound_local_kernel<true, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
+                else compound_local_kernel<false, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
+            });
         }
         if (E->params.enable_solvents && E->NS > 0) {
             out6[3] = timeit([&] {
@@ -1933,33 +1947,52 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 return (double)ms / reps;
             };
             const bool fx = E->params.accumulation == LIMA_ACCUM_FIXED64;
-            double tbest = 1e30;
+            // three sweeps, the fastest time of each shape: the first launches after an idle
+            // period run at lower clocks, and single sweeps picked different shapes run to run
+            constexpr int kSweeps = 3;
             if (E->params.enable_solvents && E->NS > 0 && E->NCH > 0) {
-                for (int spt : {1, 2, 4}) {
-                    for (int chunk : {256, 512, 1024, 2048}) {
-                        E->cfg.spt = spt;
-                        E->cfg.chunk_particles = chunk;
-                        E->build_scwork();
-                        E->reps.upload(E->reps_h);
-                        StepArgs a = E->step_args(E->cur);
-                        const double t = timeit([&] {
-                            if (fx) E->launch_solvent_chain<true, false>(a, E->stream);
-                            else E->launch_solvent_chain<false, false>(a, E->stream);
-                        });
-                        if (t < tbest) { tbest = t; best.spt = spt; best.chunk_particles = chunk; }
+                const int spts[3] = {1, 2, 4}, chunks[4] = {256, 512, 1024, 2048};
+                double tmin[3][4];
+                for (auto& row : tmin)
+                    for (double& v : row) v = 1e30;
+                for (int sweep = 0; sweep < kSweeps; sweep++) {
+                    for (int i = 0; i < 3; i++) {
+                        for (int j = 0; j < 4; j++) {
+                            E->cfg.spt = spts[i];
+                            E->cfg.chunk_particles = chunks[j];
+                            E->build_scwork();
+                            E->reps.upload(E->reps_h);
+                            StepArgs a = E->step_args(E->cur);
+                            const double t = timeit([&] {
+                                if (fx) E->launch_solvent_chain<true, false>(a, E->stream);
+                                else E->launch_solvent_chain<false, false>(a, E->stream);
+                            });
+                            tmin[i][j] = std::min(tmin[i][j], t);
+                        }
                     }
                 }
+                double tbest = 1e30;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 4; j++)
+                        if (tmin[i][j] < tbest) { tbest = tmin[i][j]; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
             }
             if (E->NC > 0) {
-                double gbest = 1e30;
                 StepArgs a = E->step_args(E->cur);
-                for (int grid : {1024, 2048, 4096, 8192}) {
-                    const double t = timeit([&] {
-                        if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
-                        else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
-                    });
-                    if (t < gbest) { gbest = t; best.listed_grid = grid; }
+                const int grids[4] = {1024, 2048, 4096, 8192};
+                double gmin[4] = {1e30, 1e30, 1e30, 1e30};
+                for (int sweep = 0; sweep < kSweeps; sweep++) {
+                    for (int j = 0; j < 4; j++) {
+                        const int grid = grids[j];
+                        const double t = timeit([&] {
+                            if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                            else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                        });
+                        gmin[j] = std::min(gmin[j], t);
+                    }
                 }
+                double gbest = 1e30;
+                for (int j = 0; j < 4; j++)
+                    if (gmin[j] < gbest) { gbest = gmin[j]; best.listed_grid = grids[j]; }
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);

@@ -77,6 +77,7 @@ struct StepArgs {
     const int* cw_off; const int* cw_total;
     // partial forces
     float4* partC;  I3* partC_fx;   // [slot*128 + p]
+    float4* cloc;   I3* cloc_fx;    // [n_chain] bridge + bonded + intra-compound LJ (compound_local_kernel)
     float4* partS;  I3* partS_fx;   // [chunk*n_solv + s]
     const SCWork* scwork; int n_scwork;
     int chunk_cap;                // chain particles of the largest solvent<-chain chunk (LDS tile size)
@@ -441,20 +442,20 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// compoundKernel remainder (Engine.cu:918-931, 981-1056): bridge carry-in,
+// compoundKernel's local forces (Engine.cu:918-937, 678-700): bridge carry-in,
 // bonded terms (term forces computed once, then gathered per particle in term
-// index order = the serialized scatter's order, Engine.cu:765-771), intra-LJ
-// (geometric eps, exclusions), listed partials, integration, PBC.
-// ADVANCE=false computes forces (and energies) without changing the state.
+// index order = the serialized scatter's order, Engine.cu:765-771) and the
+// intra-compound LJ (geometric eps, exclusions), summed in that order.  It
+// needs no neighbour list, so it runs beside the listed kernel; the sum goes to
+// cloc (int64 in the fixed-point form).
 // ---------------------------------------------------------------------------
-template <bool ADVANCE, bool FIXED, bool ENERGY>
-__global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
+template <bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
     __shared__ float4 pos[kCompoundBlock];
     __shared__ F3 rb[kMaxBondsPerCompound * 2];
     __shared__ F3 ra[kMaxAnglesPerCompound * 3];
     __shared__ F3 rd[kMaxDihedralsPerCompound * 4];
     __shared__ float2 tab[kMaxTypes * kMaxTypes];
-    __shared__ double red[2];
     const int c = blockIdx.x;
     const int p = threadIdx.x;
     const int r = a.comp_rep[c];
@@ -543,15 +544,74 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
     }
     if (FIXED) { fx_i += to_fixed(lx); fy_i += to_fixed(ly); fz_i += to_fixed(lz); }
     else force = f3_add(force, f3(lx, ly, lz));
+    // the sum so far, in the order above; compound_integrate_kernel adds the listed partials to it
+    if (active) {
+        if (FIXED) a.cloc_fx[gi] = I3{fx_i, fy_i, fz_i};
+        else a.cloc[gi] = make_float4(force.x, force.y, force.z, 0.f);
+    }
+    // thread 0's potE slot of the logged replica's first compound (Engine.cu:1027): the dihedral
+    // assigns it (Engine.cu:612), else its bond and angle terms
+    if (a.log_rep == r && c == rp.comp_begin && p == 0) {
+        float* o = a.log_data + (size_t)(a.step_count[r] % a.log_ring) * 10;
+        o[2] = (float)((a.enable_dihedrals && nd > 0) ? ed : eb + ea);
+    }
+    if (ENERGY) {
+        block_atomic_energy(eb, a.energy + r * kEnergyTerms + kE_bond);
+        block_atomic_energy(ea, a.energy + r * kEnergyTerms + kE_angle);
+        block_atomic_energy(ed, a.energy + r * kEnergyTerms + kE_dih);
+        block_atomic_energy(e_intra, a.energy + r * kEnergyTerms + kE_intra);
+        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compoundKernel remainder (Engine.cu:939-1056): the local sum plus the listed
+// partials in list order, integration, PBC.  ADVANCE=false computes forces
+// (and the kinetic energy) without changing the state.
+// ---------------------------------------------------------------------------
+template <bool ADVANCE, bool FIXED, bool ENERGY>
+__global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
+    __shared__ float4 pos[kCompoundBlock];
+    __shared__ double red[2];
+    const int c = blockIdx.x;
+    const int p = threadIdx.x;
+    const int r = a.comp_rep[c];
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half;
+    const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
+    const bool active = p < n;
+    const int gi = base + (active ? p : 0);
+    float4 me = a.cpos[gi];
+    const float4 p0 = a.cpos[base];
+    hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);                 // Engine.cu:923
+    F3 force = f3(0.f, 0.f, 0.f);
+    long long fx_i = 0, fy_i = 0, fz_i = 0;
+    if (active) {
+        if (FIXED) { const I3 v = a.cloc_fx[gi]; fx_i = v.x; fy_i = v.y; fz_i = v.z; }
+        else { const float4 v = a.cloc[gi]; force = f3(v.x, v.y, v.z); }
+    }
     // listed partials: neighbour compounds (list order), then solvent chunks
     const bool logged = ADVANCE && a.log_rep == r;
     F3 f_sol = f3(-1.f, -1.f, -1.f);   // computeSolventToCompoundLJForces' result, Float3(-1) when unset (Engine.cu:919)
     if (active) {
         const int s0 = a.cw_off[c], s1 = a.cw_off[c + 1];
-        for (int s = s0; s < s1; s++) {
-            const size_t slot = (size_t)s * kCompoundBlock + p;
-            if (FIXED) { I3 v = a.partC_fx[slot]; fx_i += v.x; fy_i += v.y; fz_i += v.z; }
-            else { float4 v = a.partC[slot]; force.x += v.x; force.y += v.y; force.z += v.z; }
+        // eight slots' loads in flight at once, then added in slot order
+        for (int s = s0; s < s1; s += 8) {
+            if (FIXED) {
+                I3 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = s + u < s1 ? a.partC_fx[(size_t)(s + u) * kCompoundBlock + p] : I3{0, 0, 0};
+#pragma unroll
+                for (int u = 0; u < 8; u++) { fx_i += v[u].x; fy_i += v[u].y; fz_i += v[u].z; }
+            } else {
+                float4 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    v[u] = s + u < s1 ? a.partC[(size_t)(s + u) * kCompoundBlock + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (s + u < s1) { force.x += v[u].x; force.y += v[u].y; force.z += v[u].z; }
+            }
         }
         if (logged && a.enable_solvents) {
             long long sx_i = 0, sy_i = 0, sz_i = 0;
@@ -600,13 +660,12 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
             }
             if (cl == 0 && p == 0) {
                 // outdata of compound 0, particle 0 (Engine.cu:1019-1028): speed, kinetic energy of the
-                // one-step-apart positions, thread 0's potE slot (the dihedral assigns it), |F|
+                // one-step-apart positions, |F| (thread 0's potE slot comes from compound_local_kernel)
                 float bx = qx, by = qy, bz = qz;
                 hyperpos(px, py, pz, bx, by, bz, L, half);
                 float* o = a.log_data + (size_t)ring * 10;
                 o[0] = (float)((double)exact_len(fsub(px, bx), fsub(py, by), fsub(pz, bz)) / rp.dt);
                 o[1] = ref_kinetic(px, py, pz, bx, by, bz, m, rp.dt, L, half);
-                o[2] = (float)((a.enable_dihedrals && nd > 0) ? ed : eb + ea);
                 o[3] = exact_len(force.x, force.y, force.z);
                 for (int j = 4; j < 10; j++) o[j] = 0.f;
             }
@@ -633,13 +692,6 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
         if ((p & 63) == 0) red[p >> 6] = ks;
         __syncthreads();
         if (p == 0) atomicAdd(a.energy + r * kEnergyTerms + kE_kin, red[0] + (blockDim.x > 64 ? red[1] : 0.0));
-    }
-    if (ENERGY) {
-        block_atomic_energy(eb, a.energy + r * kEnergyTerms + kE_bond);
-        block_atomic_energy(ea, a.energy + r * kEnergyTerms + kE_angle);
-        block_atomic_energy(ed, a.energy + r * kEnergyTerms + kE_dih);
-        block_atomic_energy(e_intra, a.energy + r * kEnergyTerms + kE_intra);
-        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
     }
 }
 
@@ -779,10 +831,23 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
     float sx = img.x, sy = img.y, sz = img.z;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     long long ix = 0, iy = 0, iz = 0;
-    for (int k = 0; k < rp.chunk_count; k++) {
-        const size_t slot = (size_t)k * a.n_solv + si;
-        if (FIXED) { I3 v = a.partS_fx[slot]; ix += v.x; iy += v.y; iz += v.z; }
-        else { float4 v = a.partS[slot]; fx += v.x; fy += v.y; fz += v.z; }
+    // eight chunks' loads in flight at once, then added in chunk order
+    for (int k = 0; k < rp.chunk_count; k += 8) {
+        if (FIXED) {
+            I3 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = k + u < rp.chunk_count ? a.partS_fx[(size_t)(k + u) * a.n_solv + si] : I3{0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < 8; u++) { ix += v[u].x; iy += v[u].y; iz += v[u].z; }
+        } else {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = k + u < rp.chunk_count ? a.partS[(size_t)(k + u) * a.n_solv + si] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k + u < rp.chunk_count) { fx += v[u].x; fy += v[u].y; fz += v[u].z; }
+        }
     }
     const float4 l = a.ss_part[si];
     if (FIXED) {

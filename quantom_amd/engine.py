@@ -263,7 +263,7 @@ class Engine:
         return {"device_us": t[0], "rebuild_us": t[1], "host_us": t[2], "steps": int(t[3]), "rebuilds": int(t[4])}
 
     KERNELS = ("solvent_chain", "compound_listed", "compound_integrate", "solvent_integrate", "bridge", "rebuild",
-               "solvent_listed", "reserved")
+               "solvent_listed", "compound_local")
 
     def time_kernels(self, reps: int = 20) -> dict:
         """Average microseconds per launch of each step kernel, launched alone (HIP events)."""
