@@ -124,32 +124,31 @@ LIMA_DEV void block_atomic_energy(double v, double* dst) {
 // compoundBridgeKernel (Engine.cu:1168-1223): one thread per bridge, terms in
 // index order, forces stored (not added) into the carry-in buffer.
 // ---------------------------------------------------------------------------
-template <bool ENERGY>
-__global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.n_bridges) return;
+// The terms of bridge b in index order; f[k] is the force on the bridge's particle k
+// (bonds, then angles, then dihedrals summed per particle, Engine.cu:1192-1211).
+constexpr int kBridgeParticles = 32;
+
+LIMA_DEV void bridge_terms(const StepArgs& a, int b, F3* f, double* eb, double* ea, double* ed) {
     const ReplicaInfo rp = a.reps[a.br_rep[b]];
-    const int p0 = a.br_off[b], np = a.br_off[b + 1] - p0;
-    constexpr int MAXP = 32;
-    F3 pos[MAXP], ps[MAXP], as[MAXP], ds[MAXP];
-    for (int k = 0; k < np && k < MAXP; k++) {
+    const int p0 = a.br_off[b], np = min(a.br_off[b + 1] - p0, kBridgeParticles);
+    F3 pos[kBridgeParticles], ps[kBridgeParticles], as[kBridgeParticles], ds[kBridgeParticles];
+    for (int k = 0; k < np; k++) {
         float4 q = a.cpos[a.br_glob[p0 + k]];
         pos[k] = f3(q.x, q.y, q.z);
         ps[k] = as[k] = ds[k] = f3(0.f, 0.f, 0.f);
     }
-    for (int k = 0; k < np && k < MAXP; k++) hyperpos(pos[0].x, pos[0].y, pos[0].z, pos[k].x, pos[k].y, pos[k].z, rp.L, rp.half);
-    double eb = 0, ea = 0, ed = 0;
+    for (int k = 0; k < np; k++) hyperpos(pos[0].x, pos[0].y, pos[0].z, pos[k].x, pos[k].y, pos[k].z, rp.L, rp.half);
     for (int t = a.brb_off[b]; t < a.brb_off[b + 1]; t++) {
         BondD bd = a.brb[t];
         F3 r[2];
-        pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, &eb);
+        pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, eb);
         ps[bd.a0] = f3_add(ps[bd.a0], r[0]);
         ps[bd.a1] = f3_add(ps[bd.a1], r[1]);
     }
     for (int t = a.bra_off[b]; t < a.bra_off[b + 1]; t++) {
         AngleD ad = a.bra[t];
         F3 r[3];
-        anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, &ea);
+        anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, ea);
         as[ad.a0] = f3_add(as[ad.a0], r[0]);
         as[ad.a1] = f3_add(as[ad.a1], r[1]);
         as[ad.a2] = f3_add(as[ad.a2], r[2]);
@@ -158,17 +157,27 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
         for (int t = a.brd_off[b]; t < a.brd_off[b + 1]; t++) {
             DihD dd = a.brd[t];
             F3 r[4];
-            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, &ed);
+            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, ed);
             ds[dd.a0] = f3_add(ds[dd.a0], r[0]);
             ds[dd.a1] = f3_add(ds[dd.a1], r[1]);
             ds[dd.a2] = f3_add(ds[dd.a2], r[2]);
             ds[dd.a3] = f3_add(ds[dd.a3], r[3]);
         }
     }
-    for (int k = 0; k < np && k < MAXP; k++) {
-        F3 f = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps[k]), as[k]), ds[k]);
-        a.fbridge[a.br_glob[p0 + k]] = make_float4(f.x, f.y, f.z, 0.0f);
-    }
+    for (int k = 0; k < np; k++) f[k] = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps[k]), as[k]), ds[k]);
+}
+
+// compoundBridgeKernel (Engine.cu:1168-1223), one thread per bridge: the carry-in
+// forces stored (not added) into fbridge, and the bridge energies.
+template <bool ENERGY>
+__global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.n_bridges) return;
+    F3 f[kBridgeParticles];
+    double eb = 0, ea = 0, ed = 0;
+    bridge_terms(a, b, f, &eb, &ea, &ed);
+    const int p0 = a.br_off[b], np = min(a.br_off[b + 1] - p0, kBridgeParticles);
+    for (int k = 0; k < np; k++) a.fbridge[a.br_glob[p0 + k]] = make_float4(f[k].x, f[k].y, f[k].z, 0.0f);
     if (ENERGY) {
         double* e = a.energy + a.br_rep[b] * kEnergyTerms;
         atomicAdd(e + kE_bond, eb);
