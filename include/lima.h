@@ -238,6 +238,12 @@ int lima_get_timings(lima_engine* engine, double* out5);
 int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
 /* per-kernel launch shape chosen for this engine (see lima_autotune) */
 int lima_get_config(lima_engine* engine, int32_t* out8);
+/* Picks the launch shape (solvents per lane and chunk size of the all-pairs
+   kernel, grid of the listed kernel) by timing whole steps of each shape,
+   trial_steps at a time, from the current state; the state, lists and
+   thermostat scalars are restored (the last-step force buffers hold a trial
+   step's).  Choices are cached per system shape; LIMA_AUTOTUNE_CACHE names a
+   file that keeps them across processes. */
 int lima_autotune(lima_engine* engine, int32_t trial_steps);
 
 /* ---- parameters and reverse-mode gradients of trajectory losses.

@@ -1936,22 +1936,56 @@ int lima_autotune(lima_engine* E, int32_t reps) {
             if (it != g_tune_cache.end()) { best = it->second; have = true; }
         }
         if (!have) {
+            // Each shape is timed on the whole step: captured cycles of all streams, rebuilds
+            // included, from the same saved state, which is restored afterwards.  Isolated kernel
+            // timings ranked the all-pairs chunk sizes differently from the step they run in.
             hipEvent_t e0, e1;
             HIP_TRY(hipEventCreate(&e0));
             HIP_TRY(hipEventCreate(&e1));
-            auto timeit = [&](auto&& launch) -> double {
-                launch();
+            const int NCH = E->NCH, NS = E->NS, R = E->R;
+            DBuf<float4> st_save;
+            DBuf<int> lists_save;
+            st_save.alloc((size_t)2 * std::max(NCH + NS, 1));
+            lists_save.alloc(std::max<size_t>(list_words(E), 1));
+            save_state(E, st_save.p);
+            save_lists(E, lists_save.p);
+            std::vector<float> lam(R), lam_applied(R);
+            HIP_TRY(hipStreamSynchronize(E->stream));
+            HIP_TRY(hipMemcpy(lam.data(), E->lambda.p, R * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(lam_applied.data(), E->lambda_applied.p, R * sizeof(float), hipMemcpyDeviceToHost));
+            const int cur0 = E->cur;
+            const long long step0 = E->step, since0 = E->since_rebuild, nreb0 = E->n_rebuilds;
+            auto restore = [&] {
+                HIP_TRY(hipStreamSynchronize(E->stream));
+                E->cur = cur0;
+                load_state(E, st_save.p);
+                load_lists(E, lists_save.p);
+                HIP_TRY(hipStreamSynchronize(E->stream));
+                HIP_TRY(hipMemcpy(E->lambda.p, lam.data(), R * sizeof(float), hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(E->lambda_applied.p, lam_applied.data(), R * sizeof(float), hipMemcpyHostToDevice));
+                E->step = step0;
+                E->since_rebuild = since0;
+                E->n_rebuilds = nreb0;
+            };
+            const int cyc = E->cycle_steps();
+            const int timed = std::max(reps, cyc);
+            // ms per step of the current shape: one cycle to capture and warm, then the timed steps
+            auto trial = [&]() -> double {
+                if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+                E->build_scwork();
+                E->reps.upload(E->reps_h);
+                E->enqueue_steps(cyc);
                 HIP_TRY(hipEventRecord(e0, E->stream));
-                for (int i = 0; i < reps; i++) launch();
+                E->enqueue_steps(timed);
                 HIP_TRY(hipEventRecord(e1, E->stream));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                return (double)ms / reps;
+                const int st = E->check_errors();
+                restore();
+                return st == LIMA_OK ? (double)ms / timed : 1e30;
             };
-            const bool fx = E->params.accumulation == LIMA_ACCUM_FIXED64;
-            // three sweeps, the fastest time of each shape: the first launches after an idle
-            // period run at lower clocks, and single sweeps picked different shapes run to run
+            // three sweeps, the fastest time of each shape (clocks ramp after idle periods)
             constexpr int kSweeps = 3;
             if (E->params.enable_solvents && E->NS > 0 && E->NCH > 0) {
                 const int spts[3] = {1, 2, 4}, chunks[4] = {256, 512, 1024, 2048};
@@ -1963,14 +1997,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                         for (int j = 0; j < 4; j++) {
                             E->cfg.spt = spts[i];
                             E->cfg.chunk_particles = chunks[j];
-                            E->build_scwork();
-                            E->reps.upload(E->reps_h);
-                            StepArgs a = E->step_args(E->cur);
-                            const double t = timeit([&] {
-                                if (fx) E->launch_solvent_chain<true, false>(a, E->stream);
-                                else E->launch_solvent_chain<false, false>(a, E->stream);
-                            });
-                            tmin[i][j] = std::min(tmin[i][j], t);
+                            tmin[i][j] = std::min(tmin[i][j], trial());
                         }
                     }
                 }
@@ -1978,19 +2005,16 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 for (int i = 0; i < 3; i++)
                     for (int j = 0; j < 4; j++)
                         if (tmin[i][j] < tbest) { tbest = tmin[i][j]; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
+                E->cfg.spt = best.spt;
+                E->cfg.chunk_particles = best.chunk_particles;
             }
             if (E->NC > 0) {
-                StepArgs a = E->step_args(E->cur);
                 const int grids[4] = {1024, 2048, 4096, 8192};
                 double gmin[4] = {1e30, 1e30, 1e30, 1e30};
                 for (int sweep = 0; sweep < kSweeps; sweep++) {
                     for (int j = 0; j < 4; j++) {
-                        const int grid = grids[j];
-                        const double t = timeit([&] {
-                            if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
-                            else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
-                        });
-                        gmin[j] = std::min(gmin[j], t);
+                        E->cfg.listed_grid = grids[j];
+                        gmin[j] = std::min(gmin[j], trial());
                     }
                 }
                 double gbest = 1e30;

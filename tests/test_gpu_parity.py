@@ -167,3 +167,26 @@ def test_fixed_point_is_deterministic(small_system):
         runs.append(e.get_state())
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+def test_autotune_leaves_state_and_fixed_point_trajectory_unchanged(small_system, monkeypatch, tmp_path):
+    """lima_autotune times whole steps from the current state and restores it: positions and
+    neighbour lists are unchanged, and a fixed-point trajectory after tuning is bit-equal to
+    one stepped with the default launch shape (int64 sums do not depend on the shape)."""
+    monkeypatch.setenv("LIMA_AUTOTUNE_CACHE", str(tmp_path / "tune.txt"))
+    print("case: autotune restores the state", flush=True)
+    a = _engine(small_system, accumulation=1, nlist_interval=10)
+    a.step(7)                                       # mid-cycle, off the graph alignment
+    before = a.get_state()
+    lists_before = [as_sets(l) for l in a.neighbors()]
+    cfg = a.autotune(10)
+    for x, y in zip(before, a.get_state()):
+        assert np.array_equal(x, y)
+    assert [as_sets(l) for l in a.neighbors()] == lists_before
+    assert cfg["solvents_per_thread"] in (1, 2, 4) and cfg["chunk_particles"] in (256, 512, 1024, 2048)
+    b = _engine(small_system, accumulation=1, nlist_interval=10)
+    b.step(7)
+    a.step(25)
+    b.step(25)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
