@@ -35,10 +35,51 @@ def workload(name):
         return [systems.small()], "small: 1 chain x 2 compounds x 128 + 3000 solvent, 5 nm, 8 types"
     if name == "large":
         return [systems.large()], "large: 10 chains x 40 compounds x 128 + lattice solvent 33/nm^3, 18 nm, 16 types"
+    if name == "t4lysozyme":
+        from quantom_amd.abi import LimaSystem
+        s = LimaSystem.load(os.path.join(ROOT, "tests", "golden", "t4lysozyme.npz"))
+        return [s], ("t4lysozyme: the reference's demo (Demo/6lzm) as ingested: 36 compounds, 1293 heavy atoms, "
+                     "5000 waters, 7 nm, 19 types, dt 0.1 fs")
     if name == "replicas":
         return systems.replicas(256), ("replicas: 256 systems, chain 128-1024 in compounds of 64, solvent 500-6000, "
                                        "5.5-6 nm")
     raise SystemExit(f"unknown workload {name}")
+
+
+# The reference's published step time (README.md:19), quoted without naming the system;
+# the shipped demo (Environment.cpp:24-33 loads Demo/6lzm) is the system it implies.
+PUBLISHED_MS_PER_STEP = 3.34
+
+
+def run_params(name, accumulation):
+    from quantom_amd.abi import ACCUM_FIXED64, ACCUM_FP32, THERMOSTAT_REFERENCE, RunParams
+    acc = ACCUM_FIXED64 if accumulation == "fixed64" else ACCUM_FP32
+    if name == "t4lysozyme":
+        # the reference's constants: CUTOFF 0.7 nm (Constants.cuh:52), lists every 200 steps
+        # (Constants.cuh:71), the thermostat cadence of Engine::hostMaster (Engine.cu:49-75)
+        return RunParams(cutoff=0.7, nlist_interval=200, thermostat=THERMOSTAT_REFERENCE, accumulation=acc,
+                         max_solvent_neighbors=1024, max_compound_solvents=4096)
+    params = RunParams(nlist_interval=20 if name == "large" else 10, accumulation=acc)
+    if name == "large":
+        params.max_compound_solvents = 32768     # compounds confined to 0.38 L span up to ~17k list entries
+    return params
+
+
+def demo_comparison(steps, warmup):
+    """ms/step on the reference's demo system, next to the reference's published figure."""
+    from quantom_amd.engine import Engine
+    syss, desc = workload("t4lysozyme")
+    eng = Engine(syss, run_params("t4lysozyme", "fp32"))
+    eng.autotune(10)
+    eng.step(warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    eng.close()
+    return {"workload": desc, "steps": steps, "ms_per_step": ms, "ns_per_day": syss[0].dt * 1e6 * 86.4 / ms,
+            "published_ms_per_step": PUBLISHED_MS_PER_STEP, "published_over_this": PUBLISHED_MS_PER_STEP / ms,
+            "note": "published figure: unnamed NVIDIA GPU (sm_61/sm_75 build), system inferred from the shipped demo"}
 
 
 def read_traffic(kernel_tag):
@@ -313,6 +354,7 @@ def main():
                     help="default: fixed64 for the large workload, fp32 otherwise")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-demo", action="store_true", help="skip the timing of the reference's demo system")
     ap.add_argument("--kernel-reps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -340,11 +382,7 @@ def main():
     n_solv = sum(s.n_solvents for s in syss)
     if args.accumulation is None:
         args.accumulation = "fixed64" if args.workload == "large" else "fp32"
-    interval = 20 if args.workload == "large" else 10
-    params = RunParams(nlist_interval=interval,
-                       accumulation=ACCUM_FIXED64 if args.accumulation == "fixed64" else ACCUM_FP32)
-    if args.workload == "large":
-        params.max_compound_solvents = 32768     # compounds confined to 0.38 L span up to ~17k list entries
+    params = run_params(args.workload, args.accumulation)
     eng = Engine(syss, params)
     if not args.no_autotune:
         t_at = time.perf_counter()
@@ -377,9 +415,11 @@ def main():
         "metric": "ms/step & ns/day, 1 MI355X (single + 256-replica batch); adjoint grads/sec",
         "value": ns_day, "unit": "ns/day", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.accumulation == "fp32" else "f32+i64acc", "data": "synthetic",
+        "dtype": "f32" if args.accumulation == "fp32" else "f32+i64acc",
+        "data": "reference demo input, ingested" if args.workload == "t4lysozyme" else "synthetic",
         "config": {"workload": desc, "n_chain": n_chain, "n_solvents": n_solv, "n_systems": len(syss),
-                   "dt_fs": dt_fs, "nlist_interval": params.nlist_interval, "thermostat": "berendsen",
+                   "dt_fs": dt_fs, "cutoff_nm": params.cutoff, "nlist_interval": params.nlist_interval,
+                   "thermostat": {0: "none", 1: "berendsen", 2: "reference"}[params.thermostat],
                    "accumulation": args.accumulation, "launch": eng.config()},
         "roofline": roof,
         "kernels_us": kt,
@@ -393,6 +433,11 @@ def main():
             out["sequential_ms_per_step"] = ms_seq
             out["batched_over_sequential"] = ms_seq / ms
             log(f"sequential single-system engines: {ms_seq:.3f} ms per step of all replicas")
+    if args.workload == "production" and not args.no_demo:
+        eng.close()
+        out["published_comparison"] = demo_comparison(2000, 200)
+        log(f"demo: {out['published_comparison']['ms_per_step']:.4f} ms/step "
+            f"(published {PUBLISHED_MS_PER_STEP} ms/step)")
     if not args.no_cpu and len(syss) == 1:
         cb = cpu_baseline(syss, params, args.cpu_budget)
         out["cpu_baseline"] = cb
