@@ -374,7 +374,7 @@ struct lima_engine {
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s3: bridge carry-in forces, then the compounds' local forces (no list needed)
         HIP_TRY(hipStreamWaitEvent(s3, ev_fork, 0));
-        if (NB > 0) bridge_kernel<ENERGY><<<(NB + 63) / 64, 64, 0, s3>>>(a);
+        if (NB > 0) bridge_kernel<ENERGY><<<NB, 64, 0, s3>>>(a);
         if (NC > 0) compound_local_kernel<FIXED, ENERGY><<<NC, kCompoundBlock, 0, s3>>>(a);
         HIP_TRY(hipEventRecord(ev_bridge, s3));
         HIP_TRY(hipStreamWaitEvent(s1, ev_bridge, 0));
@@ -1894,7 +1894,7 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
                 else solvent_integrate_kernel<false, false, false><<<E->n_stiles, 256, 0, E->stream>>>(a);
             });
         }
-        if (E->NB > 0) out6[4] = timeit([&] { bridge_kernel<false><<<(E->NB + 63) / 64, 64, 0, E->stream>>>(a); });
+        if (E->NB > 0) out6[4] = timeit([&] { bridge_kernel<false><<<E->NB, 64, 0, E->stream>>>(a); });
         out6[5] = timeit([&] { E->launch_rebuild(E->cur); });
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -124,61 +124,82 @@ LIMA_DEV void block_atomic_energy(double v, double* dst) {
 // compoundBridgeKernel (Engine.cu:1168-1223): one thread per bridge, terms in
 // index order, forces stored (not added) into the carry-in buffer.
 // ---------------------------------------------------------------------------
-// The terms of bridge b in index order; f[k] is the force on the bridge's particle k
-// (bonds, then angles, then dihedrals summed per particle, Engine.cu:1192-1211).
+// Bridges: bonded terms spanning two compounds (CompoundBridgeCompact, Bodies.cuh:644-700),
+// at most 32 particles each (MAX_PARTICLES_IN_BRIDGE, Constants.cuh:110).
 constexpr int kBridgeParticles = 32;
 
-LIMA_DEV void bridge_terms(const StepArgs& a, int b, F3* f, double* eb, double* ea, double* ed) {
-    const ReplicaInfo rp = a.reps[a.br_rep[b]];
-    const int p0 = a.br_off[b], np = min(a.br_off[b + 1] - p0, kBridgeParticles);
-    F3 pos[kBridgeParticles], ps[kBridgeParticles], as[kBridgeParticles], ds[kBridgeParticles];
-    for (int k = 0; k < np; k++) {
-        float4 q = a.cpos[a.br_glob[p0 + k]];
-        pos[k] = f3(q.x, q.y, q.z);
-        ps[k] = as[k] = ds[k] = f3(0.f, 0.f, 0.f);
-    }
-    for (int k = 0; k < np; k++) hyperpos(pos[0].x, pos[0].y, pos[0].z, pos[k].x, pos[k].y, pos[k].z, rp.L, rp.half);
-    for (int t = a.brb_off[b]; t < a.brb_off[b + 1]; t++) {
-        BondD bd = a.brb[t];
-        F3 r[2];
-        pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, eb);
-        ps[bd.a0] = f3_add(ps[bd.a0], r[0]);
-        ps[bd.a1] = f3_add(ps[bd.a1], r[1]);
-    }
-    for (int t = a.bra_off[b]; t < a.bra_off[b + 1]; t++) {
-        AngleD ad = a.bra[t];
-        F3 r[3];
-        anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, ea);
-        as[ad.a0] = f3_add(as[ad.a0], r[0]);
-        as[ad.a1] = f3_add(as[ad.a1], r[1]);
-        as[ad.a2] = f3_add(as[ad.a2], r[2]);
-    }
-    if (a.enable_dihedrals) {
-        for (int t = a.brd_off[b]; t < a.brd_off[b + 1]; t++) {
-            DihD dd = a.brd[t];
-            F3 r[4];
-            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, ed);
-            ds[dd.a0] = f3_add(ds[dd.a0], r[0]);
-            ds[dd.a1] = f3_add(ds[dd.a1], r[1]);
-            ds[dd.a2] = f3_add(ds[dd.a2], r[2]);
-            ds[dd.a3] = f3_add(ds[dd.a3], r[3]);
-        }
-    }
-    for (int k = 0; k < np; k++) f[k] = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps[k]), as[k]), ds[k]);
-}
+// compoundBridgeKernel (Engine.cu:1168-1223), one wavefront per bridge.  The bridge's
+// particles are gathered into LDS and re-imaged against its first particle; the lanes
+// evaluate up to kBridgeChunk terms of one kind at a time, and lane k then adds the
+// results for particle k in term order (bonds, then angles, then dihedrals, each kind
+// summed separately), the order of a serial pass over the terms, so the carry-in forces
+// are the serial ones bit for bit.  They are stored (not added) into fbridge.
+constexpr int kBridgeChunk = 64;
 
-// compoundBridgeKernel (Engine.cu:1168-1223), one thread per bridge: the carry-in
-// forces stored (not added) into fbridge, and the bridge energies.
 template <bool ENERGY>
 __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.n_bridges) return;
-    F3 f[kBridgeParticles];
-    double eb = 0, ea = 0, ed = 0;
-    bridge_terms(a, b, f, &eb, &ea, &ed);
+    __shared__ F3 pos[kBridgeParticles];
+    __shared__ F3 res[kBridgeChunk * 4];
+    __shared__ int who[kBridgeChunk * 4];
+    __shared__ double en[kBridgeChunk];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const ReplicaInfo rp = a.reps[a.br_rep[b]];
     const int p0 = a.br_off[b], np = min(a.br_off[b + 1] - p0, kBridgeParticles);
-    for (int k = 0; k < np; k++) a.fbridge[a.br_glob[p0 + k]] = make_float4(f[k].x, f[k].y, f[k].z, 0.0f);
-    if (ENERGY) {
+    F3 me = f3(0.f, 0.f, 0.f);
+    if (lane < np) {
+        const float4 q = a.cpos[a.br_glob[p0 + lane]];
+        me = f3(q.x, q.y, q.z);
+        pos[lane] = me;
+    }
+    __syncthreads();
+    if (lane > 0 && lane < np) {
+        const F3 ref = pos[0];
+        hyperpos(ref.x, ref.y, ref.z, me.x, me.y, me.z, rp.L, rp.half);
+        pos[lane] = me;
+    }
+    __syncthreads();
+    F3 ps = f3(0.f, 0.f, 0.f), as = ps, ds = ps;
+    double eb = 0, ea = 0, ed = 0;
+    // one kind of term: compute a chunk into LDS, then each particle's lane adds its slots in order
+    auto sweep = [&](int t0, int nterms, int slots, auto&& eval, F3& acc, double& esum) {
+        for (int c0 = 0; c0 < nterms; c0 += kBridgeChunk) {
+            const int nt = min(kBridgeChunk, nterms - c0);
+            if (lane < nt) {
+                double e = 0;
+                eval(t0 + c0 + lane, &res[lane * 4], &who[lane * 4], &e);
+                en[lane] = e;
+            }
+            __syncthreads();
+            if (lane < np)
+                for (int t = 0; t < nt; t++)
+                    for (int k = 0; k < slots; k++)
+                        if (who[t * 4 + k] == lane) acc = f3_add(acc, res[t * 4 + k]);
+            if (ENERGY && lane == 0)
+                for (int t = 0; t < nt; t++) esum += en[t];
+            __syncthreads();
+        }
+    };
+    sweep(a.brb_off[b], a.brb_off[b + 1] - a.brb_off[b], 2, [&](int t, F3* r, int* w, double* e) {
+        const BondD bd = a.brb[t];
+        pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, e);
+        w[0] = bd.a0; w[1] = bd.a1;
+    }, ps, eb);
+    sweep(a.bra_off[b], a.bra_off[b + 1] - a.bra_off[b], 3, [&](int t, F3* r, int* w, double* e) {
+        const AngleD ad = a.bra[t];
+        anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, e);
+        w[0] = ad.a0; w[1] = ad.a1; w[2] = ad.a2;
+    }, as, ea);
+    if (a.enable_dihedrals)
+        sweep(a.brd_off[b], a.brd_off[b + 1] - a.brd_off[b], 4, [&](int t, F3* r, int* w, double* e) {
+            const DihD dd = a.brd[t];
+            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, e);
+            w[0] = dd.a0; w[1] = dd.a1; w[2] = dd.a2; w[3] = dd.a3;
+        }, ds, ed);
+    if (lane < np) {
+        const F3 f = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps), as), ds);
+        a.fbridge[a.br_glob[p0 + lane]] = make_float4(f.x, f.y, f.z, 0.0f);
+    }
+    if (ENERGY && lane == 0) {
         double* e = a.energy + a.br_rep[b] * kEnergyTerms;
         atomicAdd(e + kE_bond, eb);
         atomicAdd(e + kE_angle, ea);

@@ -390,3 +390,58 @@ def test_committed_demo_fixture_is_current(demo_build):
     s, _, _ = demo_build
     ref = LimaSystem.load(os.path.join(GOLDEN, "t4lysozyme.npz"))
     assert _same_system(s, ref) == []
+
+
+CPP_CALLER = r"""
+#include "lima_ingest.h"
+#include <cstdio>
+#include <cstdlib>
+// Environment::Environment's build (Environment.cpp:16-45) through the C ABI
+int main(int argc, char** argv) {
+    const char* prm[] = {argv[1], argv[2]};
+    if (lima_ff_merge(prm, 2, argv[5]) != LIMA_OK) { std::printf("merge %s\n", lima_ingest_last_error()); return 1; }
+    if (lima_ff_prepare(argv[3], argv[4], argv[5]) != LIMA_OK) { std::printf("prepare %s\n", lima_ingest_last_error()); return 1; }
+    lima_ingest_params ip;
+    lima_ingest_default_params(&ip);
+    ip.max_compound_particles = 8;
+    ip.max_atoms_in_residue = 6;
+    lima_ingest* in = nullptr;
+    if (lima_ingest_create(argv[3], argv[4], argv[5], &ip, &in) != LIMA_OK) { std::printf("create %s\n", lima_ingest_last_error()); return 1; }
+    lima_system s;
+    lima_ingest_system(in, &s);
+    lima_ingest_info info;
+    lima_ingest_get_info(in, &info);
+    std::printf("%d %d %d %d %lld", s.n_compounds, s.n_bridges, s.compound_offset[s.n_compounds], s.n_solvents,
+                (long long)info.n_random_draws);
+    for (int i = 0; i < 3 * s.n_solvents; i++) std::printf(" %a", s.solvent_prev_positions[i]);
+    std::printf("\n");
+    lima_ingest_destroy(in);
+    lima_ingest* bad = nullptr;
+    int st = lima_ingest_create("/nonexistent.gro", argv[4], argv[5], &ip, &bad);
+    std::printf("%d %d\n", st, bad == nullptr);
+    return 0;
+}
+"""
+
+
+def test_cpp_caller_builds_the_fixture_through_the_c_abi(tmp_path):
+    """A C++ program written against include/lima_ingest.h runs all three stages and gets
+    what the Python binding gets; a missing file is a status code, not exit()."""
+    from quantom_amd import ingest
+    from quantom_amd.abi import LIMA_E_ARG
+    src = tmp_path / "caller.cpp"
+    src.write_text(CPP_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(ingest.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-llima_ingest", f"-Wl,-rpath,{libdir}"], check=True, capture_output=True)
+    d = tmp_path / "ff"
+    d.mkdir()
+    out = subprocess.run([str(exe), PRM[0], PRM[1], GRO, TOP, str(d)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    first, second = out.stdout.strip().split("\n")
+    s, info = ingest.build_system(GRO, TOP, str(d), **SMALL)
+    f = first.split()
+    assert [int(x) for x in f[:5]] == [s.n_compounds, s.n_bridges, s.n_chain, s.n_solvents, info.n_random_draws]
+    assert [float.fromhex(x) for x in f[5:]] == [float(v) for v in s.solvent_prev_positions.ravel()]
+    assert second.split() == [str(LIMA_E_ARG), "1"]
