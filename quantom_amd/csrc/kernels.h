@@ -135,6 +135,8 @@ constexpr int kBridgeParticles = 32;
 // summed separately), the order of a serial pass over the terms, so the carry-in forces
 // are the serial ones bit for bit.  They are stored (not added) into fbridge.
 constexpr int kBridgeChunk = 64;
+static_assert(kMaxTypes * kMaxTypes % kCompoundBlock == 0 && kSolventChunk % kCompoundBlock == 0,
+              "staging loops stride by the block size");
 
 template <bool ENERGY>
 __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
@@ -145,6 +147,13 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
     const int b = blockIdx.x, lane = threadIdx.x;
     const ReplicaInfo rp = a.reps[a.br_rep[b]];
     const int p0 = a.br_off[b], np = min(a.br_off[b + 1] - p0, kBridgeParticles);
+    const int tb = a.brb_off[b], nb = a.brb_off[b + 1] - tb;
+    const int ta = a.bra_off[b], na = a.bra_off[b + 1] - ta;
+    const int td = a.brd_off[b], nd = a.enable_dihedrals ? a.brd_off[b + 1] - td : 0;
+    // each kind's first chunk of term records is loaded together with the positions
+    const BondD bd0 = lane < nb ? a.brb[tb + lane] : BondD{};
+    const AngleD ad0 = lane < na ? a.bra[ta + lane] : AngleD{};
+    const DihD dd0 = lane < nd ? a.brd[td + lane] : DihD{};
     F3 me = f3(0.f, 0.f, 0.f);
     if (lane < np) {
         const float4 q = a.cpos[a.br_glob[p0 + lane]];
@@ -161,12 +170,12 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
     F3 ps = f3(0.f, 0.f, 0.f), as = ps, ds = ps;
     double eb = 0, ea = 0, ed = 0;
     // one kind of term: compute a chunk into LDS, then each particle's lane adds its slots in order
-    auto sweep = [&](int t0, int nterms, int slots, auto&& eval, F3& acc, double& esum) {
+    auto sweep = [&](int nterms, int slots, auto&& eval, F3& acc, double& esum) {
         for (int c0 = 0; c0 < nterms; c0 += kBridgeChunk) {
             const int nt = min(kBridgeChunk, nterms - c0);
             if (lane < nt) {
                 double e = 0;
-                eval(t0 + c0 + lane, &res[lane * 4], &who[lane * 4], &e);
+                eval(c0, &res[lane * 4], &who[lane * 4], &e);
                 en[lane] = e;
             }
             __syncthreads();
@@ -179,22 +188,21 @@ __global__ __launch_bounds__(64) void bridge_kernel(StepArgs a) {
             __syncthreads();
         }
     };
-    sweep(a.brb_off[b], a.brb_off[b + 1] - a.brb_off[b], 2, [&](int t, F3* r, int* w, double* e) {
-        const BondD bd = a.brb[t];
+    sweep(nb, 2, [&](int c0, F3* r, int* w, double* e) {
+        const BondD bd = c0 == 0 ? bd0 : a.brb[tb + c0 + lane];
         pairbond(pos[bd.a0], pos[bd.a1], bd.b0, bd.kb, r, e);
         w[0] = bd.a0; w[1] = bd.a1;
     }, ps, eb);
-    sweep(a.bra_off[b], a.bra_off[b + 1] - a.bra_off[b], 3, [&](int t, F3* r, int* w, double* e) {
-        const AngleD ad = a.bra[t];
+    sweep(na, 3, [&](int c0, F3* r, int* w, double* e) {
+        const AngleD ad = c0 == 0 ? ad0 : a.bra[ta + c0 + lane];
         anglebond(pos[ad.a0], pos[ad.a1], pos[ad.a2], ad.t0, ad.k, r, e);
         w[0] = ad.a0; w[1] = ad.a1; w[2] = ad.a2;
     }, as, ea);
-    if (a.enable_dihedrals)
-        sweep(a.brd_off[b], a.brd_off[b + 1] - a.brd_off[b], 4, [&](int t, F3* r, int* w, double* e) {
-            const DihD dd = a.brd[t];
-            dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, e);
-            w[0] = dd.a0; w[1] = dd.a1; w[2] = dd.a2; w[3] = dd.a3;
-        }, ds, ed);
+    sweep(nd, 4, [&](int c0, F3* r, int* w, double* e) {
+        const DihD dd = c0 == 0 ? dd0 : a.brd[td + c0 + lane];
+        dihedral(pos[dd.a0], pos[dd.a1], pos[dd.a2], pos[dd.a3], dd.p0, dd.k, dd.n, r, e);
+        w[0] = dd.a0; w[1] = dd.a1; w[2] = dd.a2; w[3] = dd.a3;
+    }, ds, ed);
     if (lane < np) {
         const F3 f = f3_add(f3_add(f3_add(f3(0.f, 0.f, 0.f), ps), as), ds);
         a.fbridge[a.br_glob[p0 + lane]] = make_float4(f.x, f.y, f.z, 0.0f);
@@ -368,10 +376,13 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
         double en = 0.0, vir = 0.0;
         if (k < ncc) {
             if (cached_rep != r) {
-                for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
-                    const float4 v = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + t];
-                    tab[t] = lj_ab(v.x, v.y);
-                }
+                // a compile-time stride, so that the eight loads per lane are in flight together
+                float4 v[kMaxTypes * kMaxTypes / kCompoundBlock];
+#pragma unroll
+                for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++)
+                    v[u] = a.tab_inter[(size_t)r * kMaxTypes * kMaxTypes + p + u * kCompoundBlock];
+#pragma unroll
+                for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++) tab[p + u * kCompoundBlock] = lj_ab(v[u].x, v[u].y);
             }
             const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
             const int nbase = a.comp_off[nb], nn = a.comp_off[nb + 1] - nbase;
@@ -443,10 +454,14 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             const int kk = k - ncc;
             const int j0 = kk * kSolventChunk;
             const int nsrc = min(kSolventChunk, a.cs_cnt[c] - j0);
-            for (int jj = p; jj < nsrc; jj += blockDim.x) {
-                float4 q = a.spos[a.cs_ids[(size_t)c * a.cap_cs + j0 + jj]];
-                hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
-                src[jj] = q;
+#pragma unroll
+            for (int u = 0; u < kSolventChunk / kCompoundBlock; u++) {
+                const int jj = p + u * kCompoundBlock;
+                if (jj < nsrc) {
+                    float4 q = a.spos[a.cs_ids[(size_t)c * a.cap_cs + j0 + jj]];
+                    hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                    src[jj] = q;
+                }
             }
             __syncthreads();
             const float4 pv = a.tab_cs[r * kMaxTypes + tp];
@@ -498,9 +513,14 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
     const float4 p0 = a.cpos[base];
     hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);                 // Engine.cu:923
     pos[p] = me;
-    for (int t = p; t < kMaxTypes * kMaxTypes; t += blockDim.x) {
-        const float4 v = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + t];
-        tab[t] = lj_ab(v.x, v.y);
+    {
+        // a compile-time stride, so that the eight loads per lane are in flight together
+        float4 v[kMaxTypes * kMaxTypes / kCompoundBlock];
+#pragma unroll
+        for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++)
+            v[u] = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + p + u * kCompoundBlock];
+#pragma unroll
+        for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++) tab[p + u * kCompoundBlock] = lj_ab(v[u].x, v[u].y);
     }
     __syncthreads();
     double eb = 0, ea = 0, ed = 0;
@@ -508,16 +528,16 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
     const int a0 = a.angle_off[c], na = a.angle_off[c + 1] - a0;
     const int d0 = a.dih_off[c], nd = a.dih_off[c + 1] - d0;
     auto P = [&](int i) { float4 q = pos[i]; return f3(q.x, q.y, q.z); };
-    for (int t = p; t < nb; t += blockDim.x) {
+    for (int t = p; t < nb; t += kCompoundBlock) {
         BondD bd = a.bonds[b0 + t];
         pairbond(P(bd.a0), P(bd.a1), bd.b0, bd.kb, &rb[t * 2], &eb);
     }
-    for (int t = p; t < na; t += blockDim.x) {
+    for (int t = p; t < na; t += kCompoundBlock) {
         AngleD ad = a.angles[a0 + t];
         anglebond(P(ad.a0), P(ad.a1), P(ad.a2), ad.t0, ad.k, &ra[t * 3], &ea);
     }
     if (a.enable_dihedrals) {
-        for (int t = p; t < nd; t += blockDim.x) {
+        for (int t = p; t < nd; t += kCompoundBlock) {
             DihD dd = a.dihs[d0 + t];
             dihedral(P(dd.a0), P(dd.a1), P(dd.a2), P(dd.a3), dd.p0, dd.k, dd.n, &rd[t * 4], &ed);
         }
@@ -795,36 +815,43 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
     // offset register per load instead of a 64-bit address each
     const unsigned rbase = (unsigned)(wrow0 + sub) * (unsigned)a.ss_ld;
     const unsigned rstep = 4u * (unsigned)a.ss_ld;
-    for (int k0 = 0; k0 < wmax; k0 += kListChunk) {
-        // cooperative, coalesced read of 64 row segments, all loads in flight at once; a row
-        // already finished re-reads its entry 0 (no new line) instead of its unused tail
-        int v[16];
+    // cooperative, coalesced read of 64 row segments, all loads in flight at once; a row
+    // already finished re-reads its entry 0 (no new line) instead of its unused tail.  The
+    // next segment's ids are loaded while the current one is summed.
+    int v[16];
+    auto load_ids = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 16; i++)
             v[i] = a.ss_ids[rbase + (unsigned)i * rstep + (unsigned)(k0 + col < rcs[i] ? k0 + col : 0)];
+    };
+    if (wmax > 0) load_ids(0);
+    for (int k0 = 0; k0 < wmax; k0 += kListChunk) {
 #pragma unroll
         for (int i = 0; i < 16; i++) st[(4 * i + sub) * kListStride + col] = v[i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (k0 + kListChunk < wmax) load_ids(k0 + kListChunk);
         const int n = min(kListChunk, cnt - k0);   // this lane's entries in the segment (may be <= 0)
         const int* mine = st + lane * kListStride;
-        int k = 0;
-        for (; k + 4 <= n; k += 4) {
-            const int4 i4 = *reinterpret_cast<const int4*>(mine + k);
-            const float4 q[4] = {a.spos[i4.x], a.spos[i4.y], a.spos[i4.z], a.spos[i4.w]};
+        // all of the segment's neighbour positions in flight at once; slots past the row's
+        // count read the solvent's own position and are not summed
+        float4 q[kListChunk];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < kListChunk; u += 4) {
+            const int4 i4 = *reinterpret_cast<const int4*>(mine + u);
+            q[u + 0] = a.spos[u + 0 < n ? i4.x : si];
+            q[u + 1] = a.spos[u + 1 < n ? i4.y : si];
+            q[u + 2] = a.spos[u + 2 < n ? i4.z : si];
+            q[u + 3] = a.spos[u + 3 < n ? i4.w : si];
+        }
+#pragma unroll
+        for (int u = 0; u < kListChunk; u++) {
+            if (u < n) {
                 hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
                 if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
                 else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
             }
-        }
-        for (; k < n; k++) {
-            const float4 q = a.spos[mine[k]];
-            hyperpos(q.x, q.y, q.z, sx, sy, sz, L, half);
-            if (ENERGY) lj_acc_energy(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz, en, vir);
-            else lj_acc(sx, sy, sz, q.x, q.y, q.z, pss.x, pss.y, lx, ly, lz);
         }
         // every lane is done with the stage before the next segment overwrites it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
