@@ -337,7 +337,8 @@ def main():
     ap.add_argument("--mode", default="bench", choices=["bench", "accuracy", "copy", "adjoint"])
     ap.add_argument("--adj-frames", type=int, default=200)
     ap.add_argument("--adj-replicas", type=int, default=64)
-    ap.add_argument("--adj-checkpoint", type=int, default=20, help="frames per checkpointed segment")
+    ap.add_argument("--adj-checkpoint", type=int, default=0,
+                    help="frames per checkpointed segment (0: one segment, nothing recomputed, when it fits)")
     ap.add_argument("--adj-fd", type=int, default=12, help="parameters of replica 0 checked by FP64 differences")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--copy-reps", type=int, default=10)

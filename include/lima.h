@@ -269,7 +269,9 @@ int lima_set_parameters(lima_engine* engine, int32_t system, const double* param
  * on frame k.  NULL weights or targets read as zeros. */
 typedef struct lima_adjoint_spec {
     int32_t n_frames;                  /* K >= 1 */
-    int32_t checkpoint_interval;       /* frames per stored state; 0 = 20.  Segments start on list rebuilds. */
+    int32_t checkpoint_interval;       /* frames per stored state; segments start on list rebuilds.  0: one
+                                          segment when its frames, residuals and lists fit in half the free
+                                          device memory, else 20.  Negative values are an error. */
     const double* energy_weight;       /* [n_systems] alpha */
     const double* force_weight;        /* [n_systems] beta */
     const double* energy_target;       /* [K][n_systems] E* (J/mol) */
@@ -278,7 +280,8 @@ typedef struct lima_adjoint_spec {
 } lima_adjoint_spec;
 
 /* Forward pass over the K frames, then the adjoint recurrence backwards with
- * checkpoints and recomputation of each segment (neighbour lists restored
+ * checkpoints: the last segment's frames are kept from the forward pass, each
+ * earlier segment is recomputed from its checkpoint (neighbour lists restored
  * exactly).  loss: [n_systems]; grad: [n_systems][LIMA_N_PARAMETERS] or NULL
  * (loss only).  The engine's state, lists and step count are left as they were. */
 int lima_adjoint_run(lima_engine* engine, const lima_adjoint_spec* spec, double* loss, double* grad);

@@ -53,7 +53,7 @@ class TrajectoryLoss(torch.autograd.Function):
 
 def trajectory_loss(engine: Engine, params: torch.Tensor, n_frames: int, energy_weight=None, force_weight=None,
                     energy_target=None, chain_force_target=None, solvent_force_target=None,
-                    checkpoint_interval: int = 20) -> torch.Tensor:
+                    checkpoint_interval: int = 0) -> torch.Tensor:
     """Differentiable loss per system; `params` is [68] (one system) or [R, 68]."""
     R = len(engine.simulation.systems)
     if params.numel() != R * N_PARAMETERS:

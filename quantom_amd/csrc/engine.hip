@@ -1075,11 +1075,26 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
     const int R = E->R, NCH = E->NCH, NS = E->NS, N = NCH + NS, NC = E->NC;
     const bool solv = P.enable_solvents && NS > 0;
     const int iv = P.nlist_interval;
-    const int cpi = sp.checkpoint_interval > 0 ? sp.checkpoint_interval : 20;
+    if (sp.checkpoint_interval < 0) throw LimaError(LIMA_E_ARG, "checkpoint_interval must be >= 0");
     HIP_TRY(hipSetDevice(E->device));
     HIP_TRY(hipStreamSynchronize(E->stream));
     adj_setup(E);
     AdjBufs& B = E->adjb;
+    const long long since0 = E->since_rebuild;
+    auto rebuild_after = [&](int j) { return iv > 0 && (since0 + j + 1) % iv == 0; };
+    const size_t LW = std::max<size_t>(list_words(E), 1), CW = std::max<size_t>(cc_words(E), 1);
+    // checkpoint_interval 0: one segment (nothing recomputed) when its frames, residuals and lists
+    // take at most half of the free device memory, else segments of 20 frames
+    int cpi = sp.checkpoint_interval;
+    if (cpi == 0) {
+        int rebuilds = 0;
+        for (int j = 0; j + 1 < K; j++) rebuilds += rebuild_after(j) ? 1 : 0;
+        const size_t need = (size_t)K * std::max(N, 1) * 2 * sizeof(float4) + (size_t)(1 + rebuilds) * LW * sizeof(int);
+        const size_t held = B.X.n * sizeof(float4) + B.Rs.n * sizeof(float4) + B.lists.n * sizeof(int);
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        cpi = need <= held + free_b / 2 ? K : 20;
+    }
 
     // the symmetric-pair terms need symmetric compound lists (the engine builds them so)
     if (NC > 0) {
@@ -1095,8 +1110,6 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
     }
 
     const SavedRun saved{E->cur, E->step, E->since_rebuild, E->n_rebuilds};
-    const long long since0 = E->since_rebuild;
-    auto rebuild_after = [&](int j) { return iv > 0 && (since0 + j + 1) % iv == 0; };
     // segment starts
     std::vector<int> starts{0};
     for (int j = 0; j + 1 < K; j++) {
@@ -1112,7 +1125,6 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
         for (int j = k0; j + 1 < k1; j++) inner += rebuild_after(j) ? 1 : 0;
         max_inner = std::max(max_inner, inner);
     }
-    const size_t LW = std::max<size_t>(list_words(E), 1), CW = std::max<size_t>(cc_words(E), 1);
     ensure(B.ck, (size_t)nseg * 2 * std::max(N, 1));
     ensure(B.ck_cc, (size_t)nseg * CW);
     ensure(B.lists, (size_t)(1 + max_inner) * LW);
@@ -1137,16 +1149,29 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
     };
 
     // ---- forward pass: checkpoints, energies, force loss ----
+    // The last segment is the first one reversed: when a gradient is wanted, its frames,
+    // residuals and interior lists are kept here instead of being recomputed.
+    std::vector<int> kept_period_end;
     save_lists(E, B.lists.p);
     for (int s = 0; s < nseg; s++) {
         const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
+        const bool keep = grad_out && s == nseg - 1;
         save_state(E, B.ck.p + (size_t)s * 2 * N);
         for (int j = k0; j < k1; j++) {
+            if (keep) {
+                float4* X = B.X.p + (size_t)(j - k0) * N;
+                d2d(X, E->cpos[E->cur].p, (size_t)NCH * 16, E->stream);
+                d2d(X + NCH, E->spos[E->cur].p, (size_t)NS * 16, E->stream);
+            }
             adj_step(E, true);
             d2d(B.efr.p + (size_t)j * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8, E->stream);
-            adj_residuals(E, B.Rs.p, tgt_c(j), tgt_s(j), true, solv);
+            adj_residuals(E, keep ? B.Rs.p + (size_t)(j - k0) * N : B.Rs.p, tgt_c(j), tgt_s(j), true, solv);
             if (j + 1 < K && rebuild_after(j)) {
                 if (s + 1 < nseg && starts[s + 1] == j + 1) save_cc(E, B.ck_cc.p + (size_t)(s + 1) * CW);
+                if (keep) {
+                    save_lists(E, B.lists.p + (size_t)(1 + kept_period_end.size()) * LW);
+                    kept_period_end.push_back(j);
+                }
                 E->launch_rebuild(E->cur);
                 E->since_rebuild = 0;
             }
@@ -1177,18 +1202,23 @@ int adjoint_run(lima_engine* E, const lima_adjoint_spec& sp, double* loss_out, d
         HIP_TRY(hipMemsetAsync(B.gpar.p, 0, (size_t)R * kAdjStride * 8, E->stream));
         for (int s = nseg - 1; s >= 0; s--) {
             const int k0 = starts[s], k1 = s + 1 < nseg ? starts[s + 1] : K;
-            load_state(E, B.ck.p + (size_t)s * 2 * N);
-            if (k0 == 0 || iv <= 0) {
-                load_lists(E, B.lists.p);
-                E->since_rebuild = iv > 0 ? (since0 + k0) % iv : 0;
+            std::vector<int> period_end;
+            if (s == nseg - 1) {
+                // kept by the forward pass; the device holds the segment's last lists
+                period_end = kept_period_end;
             } else {
-                load_cc(E, B.ck_cc.p + (size_t)s * CW);
-                E->launch_rebuild(E->cur);
-                E->since_rebuild = 0;
+                load_state(E, B.ck.p + (size_t)s * 2 * N);
+                if (k0 == 0 || iv <= 0) {
+                    load_lists(E, B.lists.p);
+                    E->since_rebuild = iv > 0 ? (since0 + k0) % iv : 0;
+                } else {
+                    load_cc(E, B.ck_cc.p + (size_t)s * CW);
+                    E->launch_rebuild(E->cur);
+                    E->since_rebuild = 0;
+                }
             }
             // recompute the segment
-            std::vector<int> period_end;
-            for (int j = k0; j < k1; j++) {
+            for (int j = k0; j < k1 && s != nseg - 1; j++) {
                 float4* X = B.X.p + (size_t)(j - k0) * N;
                 d2d(X, E->cpos[E->cur].p, (size_t)NCH * 16, E->stream);
                 d2d(X + NCH, E->spos[E->cur].p, (size_t)NS * 16, E->stream);

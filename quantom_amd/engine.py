@@ -306,7 +306,7 @@ class Engine:
         return cf, sf, pot
 
     def adjoint(self, n_frames: int, energy_weight=None, force_weight=None, energy_target=None,
-                chain_force_target=None, solvent_force_target=None, checkpoint_interval: int = 20,
+                chain_force_target=None, solvent_force_target=None, checkpoint_interval: int = 0,
                 gradient: bool = True):
         """Loss per system and, with `gradient`, dLoss/dparameters [n_systems, 68] (state unchanged)."""
         sim = self.simulation

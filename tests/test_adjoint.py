@@ -222,9 +222,15 @@ def test_adjoint_checkpoint_interval_does_not_change_result(tiny_system, problem
     l1, g1 = e.adjoint(*args, checkpoint_interval=10)
     l2, g2 = e.adjoint(*args, checkpoint_interval=1000)
     l3, g3 = e.adjoint(*args, checkpoint_interval=10)
+    l4, g4 = e.adjoint(*args, checkpoint_interval=0)     # automatic: one segment, nothing recomputed
     np.testing.assert_allclose(l1, l2, rtol=1e-12)
     np.testing.assert_allclose(g1, g2, rtol=1e-5, atol=1e-6 * np.abs(g1).max())
     np.testing.assert_allclose(g1, g3, rtol=1e-5, atol=1e-6 * np.abs(g1).max())
+    np.testing.assert_allclose(l4, l2, rtol=1e-12)
+    np.testing.assert_allclose(g4, g2, rtol=1e-9, atol=1e-9 * np.abs(g2).max())
+    from quantom_amd.engine import LimaStatusError
+    with pytest.raises(LimaStatusError):
+        e.adjoint(*args, checkpoint_interval=-1)
     e.close()
 
 
