@@ -236,6 +236,18 @@ int lima_get_timings(lima_engine* engine, double* out5);
    [4] bridge, [5] full neighbour-list rebuild, [6] solvent listed LJ,
    [7] compound local forces (bridge carry-in, bonded, intra-compound LJ). */
 int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
+
+/* In-step timing of the all-pairs kernel (solvent_chain_kernel, the step's dominant
+ * kernel; solventForceKernel's compound loop, Engine.cu:1099-1117).  With on != 0 one step
+ * per neighbour-list period (the second after each rebuild; every 8th step without
+ * rebuilds) adds, on the device's constant-rate clock, the span from the kernel's first
+ * block start to its last block end, as it runs among the step's other kernels, to a
+ * running total (cleared when the clock is switched on; the launch graph is re-captured).
+ * lima_get_kernel_clock returns the total in microseconds and the launches counted; the
+ * difference of two reads is the mean over the steps between them.  No counterpart in the
+ * reference (its Engine::timings, Engine.cuh:123, are host timers per kernel group). */
+int lima_set_kernel_clock(lima_engine* engine, int32_t on);
+int lima_get_kernel_clock(lima_engine* engine, double* total_us, int64_t* launches);
 /* per-kernel launch shape chosen for this engine (see lima_autotune) */
 int lima_get_config(lima_engine* engine, int32_t* out8);
 /* Picks the launch shape (solvents per lane and chunk size of the all-pairs

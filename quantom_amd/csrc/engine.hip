@@ -155,6 +155,10 @@ struct lima_engine {
     int log_rep = -1, log_slots = 0, log_ring = 1;
     bool log_train = false;
     DBuf<float> log_data, train_data;
+    // in-step timing of the all-pairs kernel (lima_set_kernel_clock)
+    bool kclock_on = false;
+    bool kclock_step = false;     // the step being enqueued is one the clock samples
+    DBuf<unsigned long long> kclock;
     DBuf<uint32_t> err;
     DBuf<int> cell_cnt, cell_start, cell_cursor, cell_ids, solv_cell, cell_rep;
     DBuf<float4> ss_part, ss_img;
@@ -252,6 +256,7 @@ struct lima_engine {
         a.log_slots = log_slots;
         a.log_ring = log_ring;
         a.step_count = step_count.p;
+        a.kclock = kclock_on && kclock_step ? kclock.p : nullptr;
         a.log_data = log_data.p;
         a.train_data = log_rep >= 0 && log_train ? train_data.p : nullptr;
         a.energy = energy.p;
@@ -400,7 +405,8 @@ struct lima_engine {
         if (!params.enable_solvents && NS > 0)
             HIP_TRY(hipMemcpyAsync(spos[parity ^ 1].p, spos[parity].p, NS * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
-                                                temperature.p, step_count.p, params.thermostat);
+                                                temperature.p, step_count.p, params.thermostat,
+                                                kclock_on && kclock_step ? kclock.p : nullptr);
     }
 
     // the rebuild forks too: compound lists on s1, the solvent cell pipeline on the main stream
@@ -436,6 +442,9 @@ struct lima_engine {
 
     // one step plus the rebuild it schedules (Engine::hostMaster, Engine.cu:49-75)
     void enqueue_one() {
+        // the kernel clock samples one step per list period: the second (clear of the rebuild's tail)
+        const int pick = params.nlist_interval > 1 ? 1 : 0;
+        kclock_step = params.nlist_interval > 0 ? since_rebuild == pick : step % 8 == 0;
         launch_step(cur);
         cur ^= 1;
         step++;
@@ -1880,6 +1889,7 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         StepArgs a = E->step_args(E->cur);
+        a.kclock = nullptr;   // these launches are not steps: they stay out of the in-step clock
         const bool fx = E->params.accumulation == LIMA_ACCUM_FIXED64;
         auto timeit = [&](auto&& launch) -> double {
             launch();   // warm
@@ -1951,6 +1961,41 @@ int lima_get_config(lima_engine* E, int32_t* out8) {
 // the listed kernel's grid, by timing each candidate alone on the current state
 // (state and lists are not modified).  Results are cached per shape histogram in
 // the process and, when LIMA_AUTOTUNE_CACHE names a file, across processes.
+int lima_set_kernel_clock(lima_engine* E, int32_t on) {
+    return guarded(E, [&] {
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        const bool want = on != 0;
+        if (want) {
+            ensure(E->kclock, (size_t)4);
+            HIP_TRY(hipMemset(E->kclock.p, 0xFF, sizeof(unsigned long long)));
+            HIP_TRY(hipMemset(E->kclock.p + 1, 0, 3 * sizeof(unsigned long long)));
+        }
+        if (want != E->kclock_on) {
+            E->kclock_on = want;
+            if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+        }
+        return LIMA_OK;
+    });
+}
+
+int lima_get_kernel_clock(lima_engine* E, double* total_us, int64_t* launches) {
+    return guarded(E, [&] {
+        if (!total_us || !launches) throw LimaError(LIMA_E_ARG, "null output");
+        if (!E->kclock_on) throw LimaError(LIMA_E_ARG, "the kernel clock is off (lima_set_kernel_clock)");
+        HIP_TRY(hipSetDevice(E->device));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        unsigned long long v[4];
+        HIP_TRY(hipMemcpy(v, E->kclock.p, sizeof v, hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E->device));
+        if (khz <= 0) throw LimaError(LIMA_E_UNSUPPORTED, "the device reports no wall-clock rate");
+        *total_us = (double)v[2] * 1000.0 / khz;
+        *launches = (int64_t)v[3];
+        return LIMA_OK;
+    });
+}
+
 int lima_autotune(lima_engine* E, int32_t reps) {
     return guarded(E, [&] {
         if (reps <= 0) reps = 10;

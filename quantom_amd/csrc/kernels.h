@@ -91,6 +91,7 @@ struct StepArgs {
     int log_slots;                // particle slots per compound (the reference's MAX_COMPOUND_PARTICLES)
     int log_ring;                 // steps the device rings hold; step k writes slot k % log_ring
     const long long* step_count;  // [R] steps done before this one
+    unsigned long long* kclock;   // [4] span of the step's solvent_chain_kernel and its running sum, or null
     float* log_data;              // [log_ring][10]
     float* train_data;            // [log_ring][compounds of log_rep][log_slots][3 x Float3], or null
     // energies: [R*kEnergyTerms]
@@ -249,6 +250,14 @@ LIMA_DEV void replay_images(const StepArgs& a, int cb, int ce, float4* p0s, floa
 // compounds before the chunk (replay_images).  Per-compound partial sums match
 // computeLJForces' local sum (Engine.cu:640-651).
 // ---------------------------------------------------------------------------
+// In-step timing of the all-pairs kernel, on the device's constant-rate clock (wall_clock64):
+// kclock[0] = the earliest start of the first kClockHead blocks (dispatched first, one per
+// XCD), kclock[1] = the latest end of any block (work items differ in length, so any block
+// may be the last); the step's thermostat kernel, which runs after it on the same stream,
+// adds the span to kclock[2] and counts it in kclock[3], then clears kclock[0..1].  A clock
+// read stalls its wave, so the start is read by the first blocks only.
+constexpr int kClockHead = 8;
+
 template <int SPT, bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     // dynamic LDS sized to the largest chunk of this engine: [cap] float4 + [cap] float2
@@ -258,10 +267,11 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     __shared__ int tile_off[kChunkCompounds + 1];
     __shared__ float2 tab[kMaxTypes];
     __shared__ float4 p0s[kReplayStage];
+    const int tid = threadIdx.x;
+    if (a.kclock && tid == 0 && blockIdx.x < kClockHead) atomicMin(&a.kclock[0], (unsigned long long)wall_clock64());
     const SCWork wk = a.scwork[blockIdx.x];
     const ReplicaInfo rp = a.reps[wk.replica];
     const float L = rp.L, half = rp.half;
-    const int tid = threadIdx.x;
     if (tid < kMaxTypes) {
         const float4 v = a.tab_sc[wk.replica * kMaxTypes + tid];
         tab[tid] = lj_ab(v.x, v.y);
@@ -331,6 +341,10 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
     if (ENERGY) {
         block_atomic_energy(esum, a.energy + wk.replica * kEnergyTerms + kE_sc);
         block_atomic_energy(vsum, a.energy + wk.replica * kEnergyTerms + kE_virial);
+    }
+    if (a.kclock) {
+        __syncthreads();   // every wave of the block is done
+        if (tid == 0) atomicMax(&a.kclock[1], (unsigned long long)wall_clock64());
     }
 }
 
@@ -976,7 +990,8 @@ constexpr int kThermostatEvery = 40;   // STEPS_PER_THERMOSTAT (Constants.cuh:82
 
 __global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps, int n_rep, const double* ke_comp,
                                                       const double* ke_stile, float* lambda, float* lambda_applied,
-                                                      double* temperature, long long* step_count, int mode) {
+                                                      double* temperature, long long* step_count, int mode,
+                                                      unsigned long long* kclock) {
     const int r = blockIdx.x;
     if (r >= n_rep) return;
     const ReplicaInfo rp = reps[r];
@@ -988,6 +1003,15 @@ __global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps,
         lambda_applied[r] = lambda[r];
         const long long done = step_count[r] + 1;
         step_count[r] = done;
+        if (kclock && r == 0) {   // this step's span of the all-pairs kernel
+            const unsigned long long b = kclock[0], e = kclock[1];
+            if (b != ~0ull && e > b) {
+                kclock[2] += e - b;
+                kclock[3] += 1;
+            }
+            kclock[0] = ~0ull;
+            kclock[1] = 0ull;
+        }
         const double ntot = (double)((rp.chain_end - rp.chain_begin) + (rp.solv_end - rp.solv_begin));
         float lam = 1.0f;
         if (mode == 2) {

@@ -28,7 +28,7 @@ EXPORTS = (
     "lima_version", "lima_device_count", "lima_create", "lima_destroy", "lima_last_error", "lima_step",
     "lima_step_async", "lima_sync", "lima_rebuild_neighbors", "lima_get_state", "lima_set_state",
     "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
-    "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_get_config",
+    "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_set_kernel_clock", "lima_get_kernel_clock", "lima_get_config",
     "lima_autotune", "lima_get_parameters", "lima_set_parameters", "lima_adjoint_run", "lima_record_frames",
     "lima_step_observed", "lima_step_logged",
 )
@@ -72,6 +72,8 @@ def load_library() -> C.CDLL:
     L.lima_get_timings.argtypes = [vp, C.POINTER(C.c_double)]
     L.lima_get_config.argtypes = [vp, ip]
     L.lima_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.lima_set_kernel_clock.argtypes = [vp, C.c_int32]
+    L.lima_get_kernel_clock.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     dp = C.POINTER(C.c_double)
     L.lima_get_parameters.argtypes = [vp, C.c_int32, dp]
     L.lima_set_parameters.argtypes = [vp, C.c_int32, dp]
@@ -270,6 +272,18 @@ class Engine:
         t = (C.c_double * 8)()
         self._check(self.L.lima_time_kernels(self.h, reps, t))
         return dict(zip(self.KERNELS, list(t)))
+
+    def set_kernel_clock(self, on: bool = True):
+        """Time the all-pairs kernel inside every step from now on (re-captures the step graph)."""
+        self._check(self.L.lima_set_kernel_clock(self.h, 1 if on else 0))
+
+    def kernel_clock(self) -> tuple[float, int]:
+        """Total in-step span of the all-pairs kernel since the clock was switched on (us), and the
+        launches counted; differences of two reads give the mean over the steps between them."""
+        us = C.c_double()
+        n = C.c_int64()
+        self._check(self.L.lima_get_kernel_clock(self.h, C.byref(us), C.byref(n)))
+        return us.value, n.value
 
     def config(self) -> dict:
         c = (C.c_int32 * 8)()

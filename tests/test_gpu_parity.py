@@ -190,3 +190,28 @@ def test_autotune_leaves_state_and_fixed_point_trajectory_unchanged(small_system
     b.step(25)
     for x, y in zip(a.get_state(), b.get_state()):
         assert np.array_equal(x, y)
+
+
+def test_kernel_clock_times_the_all_pairs_kernel_inside_steps(small_system):
+    """lima_set_kernel_clock / lima_get_kernel_clock: one measured launch per list period of 10
+    steps, a span close to the kernel launched alone, and the trajectory unchanged by the timing."""
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: in-step kernel clock, small system", flush=True)
+    a = Engine([small_system], RunParams(nlist_interval=10))
+    b = Engine([small_system], RunParams(nlist_interval=10))
+    a.set_kernel_clock(True)
+    a.step(30)
+    b.step(30)
+    total, n = a.kernel_clock()
+    us = total / max(n, 1)
+    alone = a.time_kernels(10)["solvent_chain"]
+    print(f"in-step {us:.2f} us over {n} launches, alone {alone:.2f} us", flush=True)
+    assert n == 3 and us > 0
+    assert 0.3 * alone < us < 10 * alone
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+    a.set_kernel_clock(False)
+    a.step(5)
+    a.close()
+    b.close()

@@ -445,3 +445,23 @@ def test_cpp_caller_builds_the_fixture_through_the_c_abi(tmp_path):
     assert [int(x) for x in f[:5]] == [s.n_compounds, s.n_bridges, s.n_chain, s.n_solvents, info.n_random_draws]
     assert [float.fromhex(x) for x in f[5:]] == [float(v) for v in s.solvent_prev_positions.ravel()]
     assert second.split() == [str(LIMA_E_ARG), "1"]
+
+
+@needs_demo
+@pytest.mark.parametrize("kw", [dict(ignore_hydrogens=0), dict(min_residue_id=10, max_residue_id=40),
+                                dict(add_solvents=0, box_len=9.0)], ids=["with_hydrogens", "residues_10_40", "no_solvent"])
+def test_demo_product_matches_oracle_other_options(tmp_path, kw):
+    """The builder's options (Environment.cpp:21-23; ENABLE_SOLVENTS; BOX_LEN) on the demo."""
+    from quantom_amd import ingest
+    gro, top, prm = _demo_paths()
+    s, info = ingest.ingest(gro, top, prm, str(tmp_path / "p"), **kw)
+    okw = dict(kw)
+    if "ignore_hydrogens" in okw:
+        okw["ignore_hydrogens"] = bool(okw["ignore_hydrogens"])
+    if "add_solvents" in okw:
+        okw["solvents"] = bool(okw.pop("add_solvents"))
+    o, ing = _oracle_build(gro, top, prm, str(tmp_path / "o"), **okw)
+    assert _same_system(s, o) == []
+    assert info.n_random_draws == ing.n_rand
+    if kw.get("ignore_hydrogens") == 0:
+        assert s.n_chain == 2610 and s.n_compounds == 66             # every protein atom of residues <= 200
