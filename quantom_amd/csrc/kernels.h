@@ -353,7 +353,8 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
 // source chunk); sources are one neighbour compound (arithmetic eps, LJ
 // exclusions) or 256 entries of the compound's solvent list (geometric eps).
 // Sources are re-imaged against the compound's particle 0 (Engine.cu:949, 967).
-// One thread per target particle; partial forces go to the compound's slots.
+// One thread per target particle (two, each over half the sources, when the
+// compound fits one wavefront); partial forces go to the compound's slots.
 // ---------------------------------------------------------------------------
 LIMA_DEV int find_compound(const int* cw_off, int n, int w) {
     int lo = 0, hi = n;   // largest c with cw_off[c] <= w
@@ -368,6 +369,7 @@ template <bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
     __shared__ float4 src[kSolventChunk];
     __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (A, B) of lj_ab: 8-byte entries keep the per-lane reads conflict-light
+    __shared__ float4 xch[kCompoundBlock / 2];      // the second wave's partial sums of a split item
     __shared__ int cached_rep;
     const int p = threadIdx.x;
     if (p == 0) cached_rep = -1;
@@ -379,8 +381,13 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
         const ReplicaInfo rp = a.reps[r];
         const float L = rp.L, half = rp.half;
         const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
-        const bool active = p < n;
-        float4 me = a.cpos[base + (active ? p : 0)];
+        // a compound of at most one wavefront's particles: both waves take particle pi, each
+        // one contiguous half of the item's sources; the halves are added in order at the end
+        const bool split = n <= kCompoundBlock / 2;
+        const int pi = split ? (p & (kCompoundBlock / 2 - 1)) : p;
+        const int hs = split ? (p / (kCompoundBlock / 2)) : 0;
+        const bool active = pi < n;
+        float4 me = a.cpos[base + (active ? pi : 0)];
         const float4 p0 = a.cpos[base];
         hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);
         const int tp = ftype(me);
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             // LJ_Ignores of this particle against the neighbour (Engine.cu:660)
             uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
             if (active) {
-                for (int e = a.xoff[base + p]; e < a.xoff[base + p + 1]; e++) {
+                for (int e = a.xoff[base + pi]; e < a.xoff[base + pi + 1]; e++) {
                     int2 x = a.xent[e];
                     if (x.x == nb) {
                         uint32_t bit = 1u << (x.y & 31);
@@ -423,10 +430,12 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             // exclusions across compounds exist only between bridged compounds: a wave
             // with none against this neighbour takes the loop without the mask test
             const bool wave_clean = __ballot((m0 | m1 | m2 | m3) != 0u) == 0ull;
+            const int hn = split ? (nn + 1) / 2 : nn;
+            const int jlo = hs * hn, jhi = min(nn, jlo + hn);
             if (active && wave_clean) {
                 const float2* trow = tab + tp;
 #pragma unroll 4
-                for (int j = 0; j < nn; j++) {
+                for (int j = jlo; j < jhi; j++) {
                     const float4 q = src[j];
                     const float2 pr = trow[ftype(q) * kMaxTypes];
                     const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
@@ -442,8 +451,8 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             } else if (active) {
                 for (int g = 0; g * 32 < nn; g++) {
                     const uint32_t mw = g == 0 ? m0 : g == 1 ? m1 : g == 2 ? m2 : m3;
-                    const int jend = min(nn, g * 32 + 32);
-                    for (int j = g * 32; j < jend; j++) {
+                    const int jend = min(jhi, g * 32 + 32);
+                    for (int j = max(jlo, g * 32); j < jend; j++) {
                         const float4 q = src[j];
                         const float2 pr = tab[ftype(q) * kMaxTypes + tp];
                         float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
@@ -480,8 +489,10 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             __syncthreads();
             const float4 pv = a.tab_cs[r * kMaxTypes + tp];
             const float2 pr = lj_ab(pv.x, pv.y);
+            const int hn = split ? (nsrc + 1) / 2 : nsrc;
+            const int jlo = hs * hn, jhi = min(nsrc, jlo + hn);
             if (active) {
-                for (int j = 0; j < nsrc; j++) {
+                for (int j = jlo; j < jhi; j++) {
                     const float4 q = src[j];
                     if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz, en, vir);
                     else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr.x, pr.y, lx, ly, lz);
@@ -492,7 +503,18 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                 block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
             }
         }
-        if (active) {
+        if (split) {
+            if (hs == 1) xch[pi] = make_float4(lx, ly, lz, 0.f);
+            __syncthreads();
+            if (hs == 0 && active) {
+                const float4 o = xch[pi];
+                const size_t slot = (size_t)w * kCompoundBlock + pi;
+                if (FIXED)
+                    a.partC_fx[slot] = I3{to_fixed(lx) + to_fixed(o.x), to_fixed(ly) + to_fixed(o.y), to_fixed(lz) + to_fixed(o.z)};
+                else
+                    a.partC[slot] = make_float4(lx + o.x, ly + o.y, lz + o.z, 0.f);
+            }
+        } else if (active) {
             const size_t slot = (size_t)w * kCompoundBlock + p;
             if (FIXED) a.partC_fx[slot] = I3{to_fixed(lx), to_fixed(ly), to_fixed(lz)};
             else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
