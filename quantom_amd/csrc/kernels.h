@@ -1298,11 +1298,14 @@ __global__ __launch_bounds__(256) void nl_cell_sort_kernel(NlistArgs a) {
 
 // One block per home cell: the solvents of the (deduplicated) 27 neighbouring
 // cells are staged in LDS in a fixed cell order (each cell sorted by id); every
-// thread tests one home solvent against all candidates.  The pair predicate is
+// wavefront tests one home solvent against all candidates.  The pair predicate is
 // evaluated with the lower id as the static point, as the reference's add loop
 // does (Engine.cu:262-264), and compared as d^2 < bound(C + C).
-// 4032 staged candidates (79 KB with the cell tables) let two blocks share a CU.
-constexpr int kCellCandidates = 4032;
+// 3456 staged candidates (54 KB of positions, 13.5 KB of ids, 8 KB of queues)
+// let two blocks share a CU.
+constexpr int kCellCandidates = 3456;
+constexpr int kSsQueue = 256;   // per-wavefront ring of prefiltered candidate slots
+static_assert(kCellCandidates <= 65535 && kCellCandidates % 128 == 0, "16-bit slots, whole passes of 128");
 
 LIMA_DEV bool ss_member(int s, float4 me, int j, float4 q, float L, float half, float bound) {
     const bool lo_self = s < j;
@@ -1318,6 +1321,7 @@ constexpr int kSolventListThreads = 1024;
 __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel(NlistArgs a) {
     __shared__ float4 cand_pos[kCellCandidates];
     __shared__ int cand_id[kCellCandidates];
+    __shared__ unsigned short queue[kSolventListThreads / 64][kSsQueue];
     __shared__ int cell_b[27], cell_n[27], cell_off[28];
     // cells of one replica are contiguous: XCD-contiguous order keeps a replica's positions in one L2
     const int cell = xcd_item(blockIdx.x, gridDim.x);
@@ -1359,41 +1363,71 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
     __syncthreads();
     const int total = cell_off[nnb];
     if (total <= kCellCandidates) {
-        // stage the candidates in the fixed cell order (independent loads, no per-cell round trips)
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            int k = 0;
-            while (cell_off[k + 1] <= e) k++;
-            const int id = a.cell_ids[cell_b[k] + (e - cell_off[k])];
-            cand_id[e] = id;
-            cand_pos[e] = a.spos[id];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        // stage the candidates in the fixed cell order: wavefront w copies cells w, w + nw, ...
+        for (int k = wid; k < nnb; k += nw) {
+            const int b = cell_b[k], o = cell_off[k], m = cell_n[k];
+            for (int e = lane; e < m; e += 64) {
+                const int id = a.cell_ids[b + e];
+                cand_id[o + e] = id;
+                cand_pos[o + e] = a.spos[id];
+            }
+        }
+        // pad to whole passes of 128 with NaN positions, which no distance test accepts
+        const int padded = (total + 127) & ~127;
+        if (threadIdx.x < padded - total) {
+            const float nan = __builtin_nanf("");
+            cand_pos[total + threadIdx.x] = make_float4(nan, nan, nan, nan);
         }
         __syncthreads();
-        // each wavefront takes home solvents (staged with the home cell) in turn; its lanes test
-        // 128 candidates per pass (two independent halves, no branches) and append the members
-        // in candidate order (ballot compaction)
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        // Each wavefront takes home solvents (staged with the home cell) in turn.  A cheap
+        // minimum-image distance, with a margin far above its rounding error against the exact
+        // re-imaged one, drops the candidates that cannot be members; the rest enter the
+        // wavefront's queue in candidate order.  Whenever 64 are queued, the exact predicate
+        // runs on them and the members are appended in that order (ballot compaction), so
+        // each list is the one a test of every candidate in order would give.
         const unsigned long long below = (1ull << lane) - 1ull;
         const int home0 = cell_off[k_home], nhome = h1 - h0;
+        const float rb = sqrtf(bound) + 1e-5f * L + 1e-6f;
+        const float pre = rb * rb;
+        unsigned short* q = queue[wid];
         for (int h = wid; h < nhome; h += nw) {
             const int s = cand_id[home0 + h];
             const float4 me = cand_pos[home0 + h];
             int* row = a.ss_ids + (size_t)s * a.ss_ld;
             int count = 0;
-            for (int e0 = 0; e0 < total; e0 += 128) {
+            unsigned head = 0, tail = 0;   // wavefront-uniform queue cursors
+            auto drain = [&](int m) {      // the exact test on the m <= 64 oldest queued slots
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // other lanes' queue writes
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const bool has = lane < m;
+                const int e = has ? (int)q[(head + lane) & (kSsQueue - 1)] : 0;
+                const int j = cand_id[e];
+                const bool mem = has & ss_member(s, me, j, cand_pos[e], L, half, bound);
+                const unsigned long long bm = __ballot(mem);
+                const int pos = count + __popcll(bm & below);
+                if (mem & (pos < a.cap_ss)) row[pos] = j;
+                count += __popcll(bm);
+                head += m;
+            };
+            auto near = [&](int e) {
+                const float4 c = cand_pos[e];
+                float dx = fabsf(c.x - me.x), dy = fabsf(c.y - me.y), dz = fabsf(c.z - me.z);
+                dx = fminf(dx, fabsf(L - dx)); dy = fminf(dy, fabsf(L - dy)); dz = fminf(dz, fabsf(L - dz));
+                return dx * dx + dy * dy + dz * dz < pre;
+            };
+            for (int e0 = 0; e0 < padded; e0 += 128) {
                 const int ea = e0 + lane, eb = ea + 64;
-                const int ia = ea < total ? ea : 0, ib = eb < total ? eb : 0;
-                const int ja = cand_id[ia], jb = cand_id[ib];
-                const float4 qa = cand_pos[ia], qb = cand_pos[ib];
-                const bool ma = (ea < total) & ss_member(s, me, ja, qa, L, half, bound);
-                const bool mb = (eb < total) & ss_member(s, me, jb, qb, L, half, bound);
+                const bool ma = near(ea), mb = near(eb);
                 const unsigned long long ba = __ballot(ma), bb = __ballot(mb);
-                const int na = __popcll(ba);
-                const int pa = count + __popcll(ba & below);
-                const int pb = count + na + __popcll(bb & below);
-                if (ma & (pa < a.cap_ss)) row[pa] = ja;
-                if (mb & (pb < a.cap_ss)) row[pb] = jb;
-                count += na + __popcll(bb);
+                const unsigned ta = tail + __popcll(ba);
+                if (ma) q[(tail + __popcll(ba & below)) & (kSsQueue - 1)] = (unsigned short)ea;
+                if (mb) q[(ta + __popcll(bb & below)) & (kSsQueue - 1)] = (unsigned short)eb;
+                tail = ta + __popcll(bb);
+                while (tail - head >= 64u) drain(64);   // at most 191 queued, 63 left
             }
+            while (tail != head) drain((int)min(64u, tail - head));
             if (lane == 0) {
                 if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
                 a.ss_cnt[s] = min(count, a.cap_ss);
