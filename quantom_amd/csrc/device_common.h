@@ -154,29 +154,35 @@ LIMA_DEV double lj_energy(float inv, float A, float B) {
     return (double)(-0.16666667f * i3 * __builtin_fmaf(0.5f * B, i3, A));
 }
 
+// |d|^2, f += d fs and the virial sum with a fixed fused-multiply-add association: left to
+// the compiler, which products fuse depends on the surrounding code, and so would the rounding
+LIMA_DEV float pair_r2(float dx, float dy, float dz) { return __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy)); }
+LIMA_DEV void pair_add(float dx, float dy, float dz, float fs, float& fx, float& fy, float& fz) {
+    fx = __builtin_fmaf(dx, fs, fx);
+    fy = __builtin_fmaf(dy, fs, fy);
+    fz = __builtin_fmaf(dz, fs, fz);
+}
+LIMA_DEV void pair_virial(float r2, float fs, double& w) { w = __builtin_fma((double)r2, (double)fs, w); }
+
 LIMA_DEV void lj_acc(float px, float py, float pz, float qx, float qy, float qz, float A, float B, float& fx,
                      float& fy, float& fz) {
-    float dx = qx - px, dy = qy - py, dz = qz - pz;
-    float r2 = dx * dx + dy * dy + dz * dz;
+    const float dx = qx - px, dy = qy - py, dz = qz - pz;
+    const float r2 = pair_r2(dx, dy, dz);
     float inv;
-    float fs = lj_scalar(r2, A, B, inv);
-    fx += dx * fs;
-    fy += dy * fs;
-    fz += dz * fs;
+    const float fs = lj_scalar(r2, A, B, inv);
+    pair_add(dx, dy, dz, fs, fx, fy, fz);
 }
 
 // same pair with FP64 energy and virial d.F accumulation
 LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, float qz, float A, float B, float& fx,
                             float& fy, float& fz, double& e, double& w) {
-    float dx = qx - px, dy = qy - py, dz = qz - pz;
-    float r2 = dx * dx + dy * dy + dz * dz;
+    const float dx = qx - px, dy = qy - py, dz = qz - pz;
+    const float r2 = pair_r2(dx, dy, dz);
     float inv;
-    float fs = lj_scalar(r2, A, B, inv);
-    fx += dx * fs;
-    fy += dy * fs;
-    fz += dz * fs;
+    const float fs = lj_scalar(r2, A, B, inv);
+    pair_add(dx, dy, dz, fs, fx, fy, fz);
     e += lj_energy(inv, A, B);
-    w += (double)r2 * (double)fs;
+    pair_virial(r2, fs, w);
 }
 
 LIMA_DEV long long to_fixed(float v) {

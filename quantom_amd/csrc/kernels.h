@@ -439,13 +439,13 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                     const float4 q = src[j];
                     const float2 pr = trow[ftype(q) * kMaxTypes];
                     const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const float r2 = pair_r2(dx, dy, dz);
                     float inv;
                     const float fs = lj_scalar(r2, pr.x, pr.y, inv);
-                    lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                    pair_add(dx, dy, dz, fs, lx, ly, lz);
                     if (ENERGY) {
                         en += lj_energy(inv, pr.x, pr.y);
-                        vir += (double)r2 * (double)fs;
+                        pair_virial(r2, fs, vir);
                     }
                 }
             } else if (active) {
@@ -456,15 +456,15 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                         const float4 q = src[j];
                         const float2 pr = tab[ftype(q) * kMaxTypes + tp];
                         float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                        float r2 = dx * dx + dy * dy + dz * dz;
+                        float r2 = pair_r2(dx, dy, dz);
                         float inv;
                         float fs = lj_scalar(r2, pr.x, pr.y, inv);
                         const bool ex = (mw >> (j & 31)) & 1u;
                         fs = ex ? 0.0f : fs;
-                        lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                        pair_add(dx, dy, dz, fs, lx, ly, lz);
                         if (ENERGY && !ex) {
                             en += lj_energy(inv, pr.x, pr.y);
-                            vir += (double)r2 * (double)fs;
+                            pair_virial(r2, fs, vir);
                         }
                     }
                 }
@@ -615,15 +615,15 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
                 const float4 q = pos[j];
                 const float2 pr = tab[ftype(q) * kMaxTypes + tp];
                 float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
+                float r2 = pair_r2(dx, dy, dz);
                 float inv;
                 float fs = lj_scalar(r2, pr.x, pr.y, inv);
                 const bool ex = (j == p) || ((mw >> (j & 31)) & 1u);
                 fs = ex ? 0.0f : fs;
-                lx += dx * fs; ly += dy * fs; lz += dz * fs;
+                pair_add(dx, dy, dz, fs, lx, ly, lz);
                 if (ENERGY && !ex) {
                     e_intra += lj_energy(inv, pr.x, pr.y);
-                    vir += (double)r2 * (double)fs;
+                    pair_virial(r2, fs, vir);
                 }
             }
         }
@@ -791,14 +791,14 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
 //
 // One wavefront per block, 64 solvents: work item = (solvent tile, quarter).  The
 // lists are rows of ss_ld ids (a multiple of kListChunk).  The wavefront reads
-// the rows of its 64 solvents kListChunk entries at a time, four rows per load
+// the rows of its 64 solvents kListedChunk entries at a time, four rows per load
 // instruction (one 64-byte segment each), and transposes them through LDS; each
 // lane then walks its row from LDS.  Every list byte is read from HBM once, in
 // whole segments, which leaves L2 to the position gathers.  Items are taken in
 // XCD-contiguous order (xcd_item), so the tiles of one replica share one L2.
 // ---------------------------------------------------------------------------
-constexpr int kListChunk = 16;                 // list entries per lane per stage
-constexpr int kListStride = kListChunk + 4;    // LDS row stride in ints (16-byte aligned, conflict-free b128 reads)
+constexpr int kListChunk = 16;                 // row length granule of the solvent lists
+constexpr int kListedChunk = 16;               // list entries per lane per segment of the listed kernel
 constexpr int kXcds = 8;
 
 // Workgroups are dispatched round-robin over the XCDs (block b runs on XCD
@@ -814,9 +814,13 @@ constexpr int kListedItems = 4;   // wavefront items per 256-solvent tile
 
 template <bool ENERGY>
 __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_items) {
+    constexpr int CH = kListedChunk;          // list entries per lane per segment
+    constexpr int RPL = 64 / CH;              // rows per load instruction
+    constexpr int STR = CH + 4;               // LDS row stride in ints (16-byte aligned rows)
+    static_assert(CH % 4 == 0 && 64 % CH == 0 && kListChunk % CH == 0, "segments tile the rows");
     // the wavefront's list stage; the replay's particle-0 stage aliases it (used before the lists)
-    __shared__ __align__(16) int stage[64 * kListStride];
-    static_assert(sizeof(stage) >= kReplayStage * sizeof(float4), "replay stage must fit the list stage");
+    constexpr int kStageInts = 64 * STR > kReplayStage * 4 ? 64 * STR : kReplayStage * 4;
+    __shared__ __align__(16) int stage[kStageInts];
     const int item = xcd_item(blockIdx.x, gridDim.x);
     if (item >= n_items) return;
     const int4 tile = a.stiles[item / kListedItems];
@@ -841,40 +845,40 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
     int* st = stage;                            // rows beyond tile.z have count 0
     int wmax = cnt;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-    const int sub = lane >> 4, col = lane & (kListChunk - 1);
-    // this lane loads entry k0 + col of rows 4 i + sub (i < 16); the list buffer has 256 spare
-    // rows, so rows past the tile (count 0) stay inside it
-    int rcs[16];
+    const int sub = lane / CH, col = lane % CH;
+    // this lane loads entry k0 + col of rows RPL i + sub (i < CH); the list buffer has 256
+    // spare rows, so rows past the tile (count 0) stay inside it
+    int rcs[CH];
 #pragma unroll
-    for (int i = 0; i < 16; i++) rcs[i] = __shfl(cnt, 4 * i + sub, 64);
+    for (int i = 0; i < CH; i++) rcs[i] = __shfl(cnt, RPL * i + sub, 64);
     // 32-bit element offsets (the buffer is under 2^32 bytes, checked at creation) keep one
     // offset register per load instead of a 64-bit address each
     const unsigned rbase = (unsigned)(wrow0 + sub) * (unsigned)a.ss_ld;
-    const unsigned rstep = 4u * (unsigned)a.ss_ld;
+    const unsigned rstep = (unsigned)RPL * (unsigned)a.ss_ld;
     // cooperative, coalesced read of 64 row segments, all loads in flight at once; a row
     // already finished re-reads its entry 0 (no new line) instead of its unused tail.  The
     // next segment's ids are loaded while the current one is summed.
-    int v[16];
+    int v[CH];
     auto load_ids = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 16; i++)
+        for (int i = 0; i < CH; i++)
             v[i] = a.ss_ids[rbase + (unsigned)i * rstep + (unsigned)(k0 + col < rcs[i] ? k0 + col : 0)];
     };
     if (wmax > 0) load_ids(0);
-    for (int k0 = 0; k0 < wmax; k0 += kListChunk) {
+    for (int k0 = 0; k0 < wmax; k0 += CH) {
 #pragma unroll
-        for (int i = 0; i < 16; i++) st[(4 * i + sub) * kListStride + col] = v[i];
+        for (int i = 0; i < CH; i++) st[(RPL * i + sub) * STR + col] = v[i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (k0 + kListChunk < wmax) load_ids(k0 + kListChunk);
-        const int n = min(kListChunk, cnt - k0);   // this lane's entries in the segment (may be <= 0)
-        const int* mine = st + lane * kListStride;
+        if (k0 + CH < wmax) load_ids(k0 + CH);
+        const int n = min(CH, cnt - k0);   // this lane's entries in the segment (may be <= 0)
+        const int* mine = st + lane * STR;
         // all of the segment's neighbour positions in flight at once; slots past the row's
         // count read the solvent's own position and are not summed
-        float4 q[kListChunk];
+        float4 q[CH];
 #pragma unroll
-        for (int u = 0; u < kListChunk; u += 4) {
+        for (int u = 0; u < CH; u += 4) {
             const int4 i4 = *reinterpret_cast<const int4*>(mine + u);
             q[u + 0] = a.spos[u + 0 < n ? i4.x : si];
             q[u + 1] = a.spos[u + 1 < n ? i4.y : si];
@@ -882,7 +886,7 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
             q[u + 3] = a.spos[u + 3 < n ? i4.w : si];
         }
 #pragma unroll
-        for (int u = 0; u < kListChunk; u++) {
+        for (int u = 0; u < CH; u++) {
             if (u < n) {
                 hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
                 if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
