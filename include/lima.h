@@ -252,7 +252,8 @@ int lima_get_kernel_clock(lima_engine* engine, double* total_us, int64_t* launch
 int lima_get_config(lima_engine* engine, int32_t* out8);
 /* Picks the launch shape (solvents per lane and chunk size of the all-pairs
    kernel, grid of the listed kernel) by timing whole steps of each shape,
-   trial_steps at a time, from the current state; the state, lists and
+   trial_steps at a time (more when that is under 8 ms, in whole list cycles),
+   from the current state; the state, lists and
    thermostat scalars are restored (the last-step force buffers hold a trial
    step's).  Choices are cached per system shape; LIMA_AUTOTUNE_CACHE names a
    file that keeps them across processes. */

@@ -2043,7 +2043,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 E->n_rebuilds = nreb0;
             };
             const int cyc = E->cycle_steps();
-            const int timed = std::max(reps, cyc);
+            int timed = std::max(reps, cyc);
             // ms per step of the current shape: one cycle to capture and warm, then the timed steps
             auto trial = [&]() -> double {
                 if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
@@ -2060,6 +2060,16 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 restore();
                 return st == LIMA_OK ? (double)ms / timed : 1e30;
             };
+            // trials of at least 8 ms each, in whole list cycles, sized from one probe of the
+            // current shape: 10-step trials (2 ms on the production system) picked a shape 27%
+            // slower than the best on a freshly started device
+            {
+                const double ms = trial();
+                if (ms > 0.0 && ms < 1e29) {
+                    const int want = (int)std::ceil(8.0 / ms);
+                    timed = std::max(timed, (want + cyc - 1) / cyc * cyc);
+                }
+            }
             // three sweeps, the fastest time of each shape (clocks ramp after idle periods)
             constexpr int kSweeps = 3;
             if (E->params.enable_solvents && E->NS > 0 && E->NCH > 0) {
