@@ -52,10 +52,21 @@ struct SCWork {
     int pad0, pad1;
 };
 
-// ---- exact FP32 arithmetic (no contraction) --------------------------------
-LIMA_DEV float fadd(float a, float b) { return __fadd_rn(a, b); }
-LIMA_DEV float fsub(float a, float b) { return __fsub_rn(a, b); }
-LIMA_DEV float fmul(float a, float b) { return __fmul_rn(a, b); }
+// ---- exact FP32 arithmetic: every operation rounded on its own --------------
+// HIP's __fadd_rn family are plain operators, which the compiler may fuse into
+// FMAs with a neighbouring product; with contraction off here they never are.
+LIMA_DEV float fadd(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+LIMA_DEV float fsub(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+LIMA_DEV float fmul(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
 
 // Float3::len (QuantomTypes.cuh:60): sqrtf((x*x + y*y) + z*z), correctly rounded
 LIMA_DEV float exact_len(float x, float y, float z) {
@@ -63,9 +74,11 @@ LIMA_DEV float exact_len(float x, float y, float z) {
 }
 
 // LIMAENG::applyHyperpos (Engine.cuh:24-30) on one coordinate
+// m + L*flag with flag 0 or 1: the product is exact, so one fused operation
+// rounds exactly as the separate multiply and add do
 LIMA_DEV float hyper1(float s, float m, float L, float half) {
-    m = fadd(m, fmul(L, (float)(fsub(s, m) > half)));
-    m = fsub(m, fmul(L, (float)(fsub(s, m) < -half)));
+    m = __builtin_fmaf(L, (float)(fsub(s, m) > half), m);
+    m = __builtin_fmaf(-L, (float)(fsub(s, m) < -half), m);
     return m;
 }
 LIMA_DEV void hyperpos(float sx, float sy, float sz, float& mx, float& my, float& mz, float L, float half) {
@@ -76,8 +89,8 @@ LIMA_DEV void hyperpos(float sx, float sy, float sz, float& mx, float& my, float
 
 // applyPBC (Engine.cu:461-466) on one coordinate
 LIMA_DEV float pbc1(float p, float L) {
-    p = fadd(p, fmul(L, (float)(p < 0.0f)));
-    p = fsub(p, fmul(L, (float)(p > L)));
+    p = __builtin_fmaf(L, (float)(p < 0.0f), p);   // exact product, as in hyper1
+    p = __builtin_fmaf(-L, (float)(p > L), p);
     return p;
 }
 

@@ -215,3 +215,44 @@ def test_kernel_clock_times_the_all_pairs_kernel_inside_steps(small_system):
     a.step(5)
     a.close()
     b.close()
+
+
+def test_solvent_lists_at_the_bound_and_across_faces():
+    """The solvent-solvent list build drops far candidates with an approximate minimum-image
+    distance before the exact predicate; pairs placed within a few ulps of the bound (C + C,
+    Engine.cu:198, 264), inside the box and across each periodic face, must come out exactly as
+    the oracle lists them."""
+    from oracle.oracle import KIND_SS, Oracle
+    from quantom_amd import systems
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: solvent pairs at the list bound, 5 nm box", flush=True)
+    s = systems.build_system(0x51A7 + 91, 1, 2, 5.0, 4, 300, compound_size=16)
+    rng = np.random.default_rng(11)
+    pts, pairs = [], []
+    for k in range(-6, 7):                       # distance 1.2 nm x (1 + k 1.2e-7)
+        for wrap in (False, True):
+            for axis in range(3):
+                u = rng.normal(size=3)
+                u[axis] = abs(u[axis]) + 2.0
+                u /= np.linalg.norm(u)
+                a = rng.uniform(1.5, 3.5, size=3)
+                if wrap:
+                    a[axis] = rng.uniform(0.01, 0.2)   # the partner lies across the face
+                b = np.mod(a - 1.2 * (1.0 + k * 1.2e-7) * u, s.box_len)
+                pairs.append((len(pts), len(pts) + 1, k))
+                pts += [a, b]
+    base = len(s.solvent_positions)
+    s.solvent_positions = np.concatenate([s.solvent_positions, np.array(pts, np.float32)]).astype(np.float32)
+    s.solvent_prev_positions = s.solvent_positions.copy()
+    p = RunParams()
+    want = Oracle(s, p, precision=32).neighbors(KIND_SS)
+    member = [(base + j) in set(want[base + i]) for i, j, _ in pairs]
+    assert any(member) and not all(member)       # the placed pairs straddle the bound
+    eng = Engine([s], p)
+    got = eng.neighbors()[2]
+    diff = [(i, sorted(set(g) ^ set(w))) for i, (g, w) in enumerate(zip(got, want)) if set(g) != set(w)]
+    for i, d in diff[:8]:
+        print(f"solvent {i}: differs in {d}", flush=True)
+    assert not diff
+    eng.close()
