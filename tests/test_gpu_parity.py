@@ -256,3 +256,42 @@ def test_solvent_lists_at_the_bound_and_across_faces():
         print(f"solvent {i}: differs in {d}", flush=True)
     assert not diff
     eng.close()
+
+
+def test_compound_solvent_lists_at_the_bound():
+    """Solvents placed within a few ulps of a compound's list radius: C + r + C from its key
+    particle, r the confining sphere (Engine.cu:146-151, 188-200), in random directions and
+    wrapped into the box.  The compound->solvent lists must equal the oracle's."""
+    from oracle.oracle import KIND_CS, KIND_SS, Oracle
+    from quantom_amd import systems
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: solvents at the compound list radius, 5 nm box", flush=True)
+    s = systems.build_system(0x51A7 + 91, 1, 2, 5.0, 4, 300, compound_size=16)
+    p = RunParams()
+    rng = np.random.default_rng(5)
+    pts, tags = [], []
+    for c in range(len(s.key_particle_index)):
+        key = s.chain_positions[s.compound_offset[c] + s.key_particle_index[c]].astype(np.float64)
+        radius = p.cutoff + float(s.confining_particle_sphere[c]) + p.cutoff
+        for k in range(-6, 7):                   # radius x (1 + k 1.2e-7)
+            for _ in range(4):
+                u = rng.normal(size=3)
+                u /= np.linalg.norm(u)
+                pts.append(np.mod(key + radius * (1.0 + k * 1.2e-7) * u, s.box_len))
+                tags.append(c)
+    base = len(s.solvent_positions)
+    s.solvent_positions = np.concatenate([s.solvent_positions, np.array(pts, np.float32)]).astype(np.float32)
+    s.solvent_prev_positions = s.solvent_positions.copy()
+    ora = Oracle(s, p, precision=32)
+    want_cs, want_ss = ora.neighbors(KIND_CS), ora.neighbors(KIND_SS)
+    member = [(base + i) in set(want_cs[c]) for i, c in enumerate(tags)]
+    assert any(member) and not all(member)       # the placed solvents straddle the radius
+    eng = Engine([s], p)
+    _, got_cs, got_ss = eng.neighbors()
+    for name, got, want in (("compound->solvent", got_cs, want_cs), ("solvent->solvent", got_ss, want_ss)):
+        diff = [(i, sorted(set(g) ^ set(w))) for i, (g, w) in enumerate(zip(got, want)) if set(g) != set(w)]
+        for i, d in diff[:8]:
+            print(f"{name} row {i}: differs in {d}", flush=True)
+        assert not diff, name
+    eng.close()
