@@ -295,3 +295,40 @@ def test_compound_solvent_lists_at_the_bound():
             print(f"{name} row {i}: differs in {d}", flush=True)
         assert not diff, name
     eng.close()
+
+
+def test_compound_compound_lists_at_the_bound():
+    """Two compounds whose key particles sit within a few ulps of the list radius that an empty
+    list adds at, C + ((r_lo + r_lo) + C) (Engine.cu:146-151, 170-186), one offset per system
+    of a 13-system batch.  Each system's compound lists must equal the oracle's."""
+    import copy
+    from oracle.oracle import KIND_CC, Oracle
+    from quantom_amd import systems
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: compound pairs at the list radius, 13 systems of 5 nm", flush=True)
+    p = RunParams()
+    C = np.float32(p.cutoff)
+    base = systems.build_system(0x51A7 + 91, 1, 2, 5.0, 4, 300, compound_size=16)
+    r0 = np.float32(base.confining_particle_sphere[0])
+    radius = float(C + ((r0 + r0) + C))
+    o1, o2 = int(base.compound_offset[1]), int(base.compound_offset[2])
+    key0 = base.chain_positions[base.compound_offset[0] + base.key_particle_index[0]].astype(np.float64)
+    key1 = base.chain_positions[o1 + base.key_particle_index[1]].astype(np.float64)
+    u = np.ones(3) / np.sqrt(3.0)                # every component under half the box
+    syss, want = [], []
+    for k in range(-6, 7):                       # radius x (1 + k 1.2e-7)
+        s = copy.deepcopy(base)
+        shift = key0 + radius * (1.0 + k * 1.2e-7) * u - key1
+        s.chain_positions[o1:o2] = (s.chain_positions[o1:o2] + shift).astype(np.float32)
+        s.chain_prev_positions[o1:o2] = (s.chain_prev_positions[o1:o2] + shift).astype(np.float32)
+        syss.append(s)
+        want.append(Oracle(s, p, precision=32).neighbors(KIND_CC))
+    linked = [1 in set(w[0]) for w in want]
+    assert any(linked) and not all(linked)       # the placed pairs straddle the radius
+    eng = Engine(syss, p)
+    got = eng.neighbors()[0]
+    for r, w in enumerate(want):                 # system r holds compounds 2r and 2r + 1
+        for c in range(2):
+            assert set(int(x) - 2 * r for x in got[2 * r + c]) == set(w[c]), (r, c)
+    eng.close()
