@@ -248,10 +248,15 @@ int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
  * reference (its Engine::timings, Engine.cuh:123, are host timers per kernel group). */
 int lima_set_kernel_clock(lima_engine* engine, int32_t on);
 int lima_get_kernel_clock(lima_engine* engine, double* total_us, int64_t* launches);
-/* per-kernel launch shape chosen for this engine (see lima_autotune) */
-int lima_get_config(lima_engine* engine, int32_t* out8);
+/* Launch shape chosen for this engine (see lima_autotune), ten values: solvents
+   per lane of the all-pairs kernel, its target chunk size, its work items, the
+   most chunks of one solvent tile, graph replay on/off, solvent cells, solvent
+   tiles, systems, the grid of the listed compound kernel, and lanes per solvent
+   of the listed solvent kernel. */
+int lima_get_config(lima_engine* engine, int32_t* out10);
 /* Picks the launch shape (solvents per lane and chunk size of the all-pairs
-   kernel, grid of the listed kernel) by timing whole steps of each shape,
+   kernel, grid of the listed compound kernel, and, in FP32, one or four lanes
+   per solvent of the listed solvent kernel) by timing whole steps of each shape,
    trial_steps at a time (more when that is under 8 ms, in whole list cycles),
    from the current state; the state, lists and
    thermostat scalars are restored (the last-step force buffers hold a trial

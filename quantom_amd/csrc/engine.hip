@@ -74,6 +74,7 @@ struct LaunchConfig {
     int chunk_particles = 1024;   // target chain particles per solvent<-chain work item
     int listed_grid = 0;          // grid of compound_listed_kernel (0 = auto)
     int use_graph = 1;
+    int listed_lanes = 1;         // lanes per solvent of the listed solvent kernel (1, or 4 in FP32)
 };
 
 // Device buffers of lima_adjoint_run / lima_record_frames, grown on demand and
@@ -375,7 +376,12 @@ struct lima_engine {
         }
         // s2: the listed solvent-solvent LJ
         HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
-        if (solv) solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles * kListedItems), 64, 0, s2>>>(a, n_stiles * kListedItems);
+        if (solv) {
+            if (!FIXED && cfg.listed_lanes == kSplitLanes)
+                solvent_listed_split_kernel<ENERGY><<<xcd_grid(n_stiles * kSplitItems), 64, 0, s2>>>(a, n_stiles * kSplitItems);
+            else
+                solvent_listed_kernel<ENERGY><<<xcd_grid(n_stiles * kListedItems), 64, 0, s2>>>(a, n_stiles * kListedItems);
+        }
         HIP_TRY(hipEventRecord(ev_ss, s2));
         // s3: bridge carry-in forces, then the compounds' local forces (no list needed)
         HIP_TRY(hipStreamWaitEvent(s3, ev_fork, 0));
@@ -1452,22 +1458,34 @@ std::string shape_key(const lima_engine* E) {
     return os.str();
 }
 
+std::string g_tune_path;
+
+// reads the cache file once per path (a process may point LIMA_AUTOTUNE_CACHE elsewhere later)
 void load_tune_file() {
-    if (g_tune_loaded) return;
-    g_tune_loaded = true;
     const char* path = std::getenv("LIMA_AUTOTUNE_CACHE");
+    const std::string p = path ? path : "";
+    if (g_tune_loaded && p == g_tune_path) return;
+    g_tune_loaded = true;
+    g_tune_path = p;
     if (!path) return;
     std::ifstream f(path);
-    std::string key;
-    LaunchConfig c;
-    while (f >> key >> c.spt >> c.chunk_particles >> c.listed_grid) g_tune_cache[key] = c;
+    std::string line;
+    while (std::getline(f, line)) {
+        // key spt chunk_particles listed_grid [listed_lanes]
+        std::istringstream is(line);
+        std::string key;
+        LaunchConfig c;
+        if (!(is >> key >> c.spt >> c.chunk_particles >> c.listed_grid)) continue;
+        if (!(is >> c.listed_lanes)) c.listed_lanes = 1;
+        g_tune_cache[key] = c;
+    }
 }
 
 void save_tune_entry(const std::string& key, const LaunchConfig& c) {
     const char* path = std::getenv("LIMA_AUTOTUNE_CACHE");
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << "\n";
+    f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << " " << c.listed_lanes << "\n";
 }
 
 // one system's energy record from the device sums (half of every directed-pair LJ sum)
@@ -1911,7 +1929,10 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
         }
         if (E->params.enable_solvents && E->NS > 0)
             out8[6] = timeit([&] {
-                solvent_listed_kernel<false><<<xcd_grid(E->n_stiles * kListedItems), 64, 0, E->stream>>>(a, E->n_stiles * kListedItems);
+                if (!fx && E->cfg.listed_lanes == kSplitLanes)
+                    solvent_listed_split_kernel<false><<<xcd_grid(E->n_stiles * kSplitItems), 64, 0, E->stream>>>(a, E->n_stiles * kSplitItems);
+                else
+                    solvent_listed_kernel<false><<<xcd_grid(E->n_stiles * kListedItems), 64, 0, E->stream>>>(a, E->n_stiles * kListedItems);
             });
         if (E->NC > 0) {
             const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
@@ -1942,17 +1963,19 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
     });
 }
 
-int lima_get_config(lima_engine* E, int32_t* out8) {
+int lima_get_config(lima_engine* E, int32_t* out10) {
     return guarded(E, [&] {
-        if (!out8) throw LimaError(LIMA_E_ARG, "null output");
-        out8[0] = E->cfg.spt;
-        out8[1] = E->cfg.chunk_particles;
-        out8[2] = (int)E->scwork_h.size();
-        out8[3] = E->kmax;
-        out8[4] = E->cfg.use_graph;
-        out8[5] = E->n_cells;
-        out8[6] = E->n_stiles;
-        out8[7] = E->R;
+        if (!out10) throw LimaError(LIMA_E_ARG, "null output");
+        out10[0] = E->cfg.spt;
+        out10[1] = E->cfg.chunk_particles;
+        out10[2] = (int)E->scwork_h.size();
+        out10[3] = E->kmax;
+        out10[4] = E->cfg.use_graph;
+        out10[5] = E->n_cells;
+        out10[6] = E->n_stiles;
+        out10[7] = E->R;
+        out10[8] = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
+        out10[9] = E->params.accumulation == LIMA_ACCUM_FIXED64 ? 1 : E->cfg.listed_lanes;
         return LIMA_OK;
     });
 }
@@ -2105,6 +2128,20 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 double gbest = 1e30;
                 for (int j = 0; j < 4; j++)
                     if (gmin[j] < gbest) { gbest = gmin[j]; best.listed_grid = grids[j]; }
+                E->cfg.listed_grid = best.listed_grid;
+            }
+            if (E->params.enable_solvents && E->NS > 0 && E->params.accumulation != LIMA_ACCUM_FIXED64) {
+                // one or four lanes per solvent in the listed solvent kernel (FP32 only: the four
+                // partial sums group the floats differently)
+                const int lanes[2] = {1, kSplitLanes};
+                double lmin[2] = {1e30, 1e30};
+                for (int sweep = 0; sweep < kSweeps; sweep++) {
+                    for (int j = 0; j < 2; j++) {
+                        E->cfg.listed_lanes = lanes[j];
+                        lmin[j] = std::min(lmin[j], trial());
+                    }
+                }
+                best.listed_lanes = lmin[1] < lmin[0] ? kSplitLanes : 1;
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);

@@ -908,6 +908,99 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
     }
 }
 
+// The same listed sum with four lanes per solvent, for systems too small to fill
+// the device (one wavefront per 16 solvents).  Every lane of a solvent's group
+// replays the whole re-imaging sequence (it is sequential by definition), and lane
+// g evaluates the pairs 4b + g, each with the image after its own entry; the four
+// partial sums are added in lane order.  The images are the reference's; the float
+// sums are grouped differently, so this form serves FP32 accumulation only.
+constexpr int kSplitLanes = 4;
+constexpr int kSplitItems = 256 / (64 / kSplitLanes);   // wavefront items per 256-solvent tile
+
+template <bool ENERGY>
+__global__ __launch_bounds__(64) void solvent_listed_split_kernel(StepArgs a, int n_items) {
+    __shared__ __align__(16) int stage[kReplayStage * 4];   // the replay's particle-0 stage, then ids
+    const int item = xcd_item(blockIdx.x, gridDim.x);
+    if (item >= n_items) return;
+    const int4 tile = a.stiles[item / kSplitItems];
+    const int lane = threadIdx.x, grp = lane / kSplitLanes, g = lane % kSplitLanes;
+    const int wrow0 = tile.y + (item % kSplitItems) * (64 / kSplitLanes);
+    if (wrow0 >= tile.z) return;
+    const int r = tile.x;
+    const ReplicaInfo rp = a.reps[r];
+    const float L = rp.L, half = rp.half;
+    const int s = wrow0 + grp;
+    const bool active = s < tile.z;
+    const int si = active ? s : tile.y;
+    const float4 me = a.spos[si];
+    float sx = me.x, sy = me.y, sz = me.z;
+    replay_images<1>(a, rp.comp_begin, rp.comp_end, reinterpret_cast<float4*>(stage), &sx, &sy, &sz, L, half);   // Engine.cu:1111
+    __syncthreads();
+    const float4 pv = a.tab_ss[r];
+    const float2 pss = lj_ab(pv.x, pv.y);
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    double en = 0, vir = 0;
+    const int cnt = active ? a.ss_cnt[si] : 0;
+    int wmax = cnt;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    const int* row = a.ss_ids + (size_t)si * a.ss_ld;
+    int4* st4 = reinterpret_cast<int4*>(stage);
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    for (int k0 = 0; k0 < wmax; k0 += 16) {
+        // lane g of a group reads entries k0 + 4g .. k0 + 4g + 3 of the row (16-byte aligned: the
+        // rows are multiples of 16 ids, and k0 < count <= row length)
+        const int4 ids = k0 < cnt ? *reinterpret_cast<const int4*>(row + k0 + 4 * g) : make_int4(si, si, si, si);
+        st4[lane] = ids;
+        wave_sync();
+        const int n = min(16, cnt - k0);
+        float4 q[16];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int4 i4 = st4[grp * kSplitLanes + b];
+            q[4 * b + 0] = a.spos[4 * b + 0 < n ? i4.x : si];
+            q[4 * b + 1] = a.spos[4 * b + 1 < n ? i4.y : si];
+            q[4 * b + 2] = a.spos[4 * b + 2 < n ? i4.z : si];
+            q[4 * b + 3] = a.spos[4 * b + 3 < n ? i4.w : si];
+        }
+        wave_sync();   // the stage is rewritten by the next segment
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            float ex = sx, ey = sy, ez = sz, qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int u = 4 * b + t;
+                if (u < n) hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
+                const bool mine = t == g;   // this lane's pair: the image after its own entry
+                ex = mine ? sx : ex; ey = mine ? sy : ey; ez = mine ? sz : ez;
+                qx = mine ? q[u].x : qx; qy = mine ? q[u].y : qy; qz = mine ? q[u].z : qz;
+            }
+            if (4 * b + g < n) {
+                if (ENERGY) lj_acc_energy(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz, en, vir);
+                else lj_acc(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz);
+            }
+        }
+    }
+    // the group's partial sums in lane order
+    const int g0 = grp * kSplitLanes;
+    float tx = __shfl(lx, g0, 64), ty = __shfl(ly, g0, 64), tz = __shfl(lz, g0, 64);
+#pragma unroll
+    for (int j = 1; j < kSplitLanes; j++) {
+        tx += __shfl(lx, g0 + j, 64); ty += __shfl(ly, g0 + j, 64); tz += __shfl(lz, g0 + j, 64);
+    }
+    if (active && g == 0) {
+        a.ss_part[s] = make_float4(tx, ty, tz, 0.f);
+        a.ss_img[s] = make_float4(sx, sy, sz, 0.f);
+    }
+    if (ENERGY) {
+        block_atomic_energy(en, a.energy + r * kEnergyTerms + kE_ss);
+        block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // solventForceKernel remainder (Engine.cu:1129-1155): force = compound
 // contributions (chunk partials in order) + the solvent-solvent local sum;

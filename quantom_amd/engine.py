@@ -286,10 +286,10 @@ class Engine:
         return us.value, n.value
 
     def config(self) -> dict:
-        c = (C.c_int32 * 8)()
+        c = (C.c_int32 * 10)()
         self._check(self.L.lima_get_config(self.h, c))
         keys = ("solvents_per_thread", "chunk_particles", "solvent_chain_items", "max_chunks", "use_graph",
-                "n_cells", "n_solvent_tiles", "n_replicas")
+                "n_cells", "n_solvent_tiles", "n_replicas", "listed_grid", "listed_lanes")
         return dict(zip(keys, list(c)))
 
     def autotune(self, trial_steps: int = 20) -> dict:

@@ -332,3 +332,40 @@ def test_compound_compound_lists_at_the_bound():
         for c in range(2):
             assert set(int(x) - 2 * r for x in got[2 * r + c]) == set(w[c]), (r, c)
     eng.close()
+
+
+def test_four_lanes_per_solvent_listed_form(small_system, monkeypatch, tmp_path):
+    """The FP32 listed solvent kernel with four lanes per solvent (chosen by the autotuner for
+    small systems): forces within the oracle bar, and after 20 steps the same images and a
+    trajectory as close to the oracle as the one-lane form's."""
+    from oracle.oracle import Oracle
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: four lanes per solvent, small system", flush=True)
+    s = small_system
+    key = f"acc0_sol1_{(s.n_chain + 64) // 128}x{(s.n_solvents + 256) // 512}:1"
+    path = tmp_path / "tune.txt"
+    path.write_text(f"{key} 2 1024 4096 4\n")
+    monkeypatch.setenv("LIMA_AUTOTUNE_CACHE", str(path))
+    p = RunParams(nlist_interval=10)
+    four = Engine([s], p)
+    assert four.autotune(10)["listed_lanes"] == 4
+    one = Engine([s], p)
+    ora = Oracle(s, p, precision=32)
+    f4 = np.concatenate([x.ravel() for x in four.compute_forces()])
+    f1 = np.concatenate([x.ravel() for x in one.compute_forces()])
+    fo = np.concatenate([x.ravel() for x in ora.compute_forces()[:2]])
+    print(f"force rel rms: four lanes {rel_rms(f4, fo):.2e}, one lane {rel_rms(f1, fo):.2e}", flush=True)
+    assert rel_rms(f4, fo) <= 1e-5
+    four.step(20)
+    one.step(20)
+    assert ora.step(20) == 0
+    g4, g1, go = four.get_state(), one.get_state(), ora.state()
+    def max_dx(x, y):   # periodic: a coordinate wrapped on one side only differs by about L
+        d = np.abs(np.concatenate([x[0].ravel() - y[0].ravel(), x[2].ravel() - y[2].ravel()]))
+        return float(np.minimum(d, s.box_len - d).max())
+    d4, d1 = max_dx(g4, go), max_dx(g1, go)
+    print(f"after 20 steps max |dx| vs oracle: four lanes {d4:.2e}, one lane {d1:.2e}", flush=True)
+    assert d4 <= 3 * d1 + 1e-5
+    four.close()
+    one.close()
