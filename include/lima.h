@@ -273,7 +273,11 @@ int lima_autotune(lima_engine* engine, int32_t trial_steps);
  *   [64] kb  [65] b0  [66] k_theta  [67] theta_0
  * The four bonded entries act as constants shared by every pair bond / angle of
  * the system (bridges included): setting them writes every term, and their
- * gradient is the sum of the per-term derivatives. */
+ * gradient is the sum of the per-term derivatives (the derivative for a common
+ * shift of every term's constant).  get returns the first term's constants.  On
+ * a system whose terms carry different constants (an ingested molecule), set
+ * keeps the per-term constants when entries 64-67 come back unchanged and
+ * returns LIMA_E_UNSUPPORTED for a change of the differing kind. */
 #define LIMA_N_PARAMETERS 68
 
 int lima_get_parameters(lima_engine* engine, int32_t system, double* params);

@@ -198,7 +198,23 @@ LIMA_DEV void lj_acc_energy(float px, float py, float pz, float qx, float qy, fl
     pair_virial(r2, fs, w);
 }
 
-LIMA_DEV long long to_fixed(float v) {
+// A partial force enters the int64 sums only while |F| < 2^40 J/(mol nm) (2^61 in fixed
+// point), so that sums of a few such partials stay inside int64; the critical force the
+// reference stops at is 2e12 (Engine.cu:1050), just under 2^41.  A NaN or infinite partial
+// raises kErrNaN and contributes 0; a finite partial at or beyond the bound raises
+// kErrCriticalForce and saturates at +-2^60.  The float-to-int64 conversion itself is
+// undefined for such values, so the test comes first.
+constexpr float kFixedPartialBound = 1099511627776.0f;   // 2^40
+LIMA_DEV long long to_fixed(float v, uint32_t& bad) {
+    const float m = fabsf(v);
+    if (!(m < kFixedPartialBound)) {
+        if (m != m || m == INFINITY) {
+            bad |= kErrNaN;
+            return 0;
+        }
+        bad |= kErrCriticalForce;
+        return v > 0.f ? (1ll << 60) : -(1ll << 60);
+    }
     return __float2ll_rn(v * (float)(1 << kFixedShift));
 }
 LIMA_DEV float from_fixed(long long v) {

@@ -173,8 +173,16 @@ struct lima_engine {
     // s3:     bridge -> compound local forces
     hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr;
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;   // lima_step's device timer
+    // Captured launch graphs: one whole rebuild cycle (replayed from its start), and one step
+    // of each kind [parity][rebuild follows][clocked] for steps that do not start a cycle.
+    // Both are captured together on the first stepping call after any change of shape.
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t step_graph[2][2][2] = {};
+    bool graphs_ready = false;
     int graph_steps = 0;
+    long long dev_step = -1;      // the device step counters' value after the queued work (-1: unknown)
+    bool async_open = false;      // ev_t0 marks the start of steps queued by lima_step_async
 
     // ---- run state ----
     int cur = 0;                  // ping-pong parity of the current state
@@ -185,8 +193,8 @@ struct lima_engine {
     std::string last_error;
 
     ~lima_engine() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c})
+        drop_graphs();
+        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c, ev_t0, ev_t1})
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t s : {stream, s1, s2, s3})
             if (s) (void)hipStreamDestroy(s);
@@ -199,6 +207,21 @@ struct lima_engine {
         HIP_TRY(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
         for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        HIP_TRY(hipEventCreate(&ev_t0));
+        HIP_TRY(hipEventCreate(&ev_t1));
+    }
+
+    // every captured graph holds the launch shape, buffers and logging switches of its capture
+    void drop_graphs() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        for (auto& a : step_graph)
+            for (auto& b : a)
+                for (hipGraphExec_t& g : b) {
+                    if (g) (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+        graphs_ready = false;
     }
 
     StepArgs step_args(int parity) {
@@ -413,6 +436,7 @@ struct lima_engine {
         thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
                                                 temperature.p, step_count.p, params.thermostat,
                                                 kclock_on && kclock_step ? kclock.p : nullptr);
+        if (dev_step >= 0) dev_step++;   // the thermostat kernel advances the device step counters
     }
 
     // the rebuild forks too: compound lists on s1, the solvent cell pipeline on the main stream
@@ -446,61 +470,117 @@ struct lima_engine {
         return (iv % 2 == 0) ? iv : 2 * iv;
     }
 
-    // one step plus the rebuild it schedules (Engine::hostMaster, Engine.cu:49-75)
-    void enqueue_one() {
-        // the kernel clock samples one step per list period: the second (clear of the rebuild's tail)
-        const int pick = params.nlist_interval > 1 ? 1 : 0;
-        kclock_step = params.nlist_interval > 0 ? since_rebuild == pick : step % 8 == 0;
-        launch_step(cur);
+    // What distinguishes one step's launches from another's: the state parity, whether the
+    // scheduled rebuild (Engine::hostMaster, Engine.cu:49-75) follows it, and whether the kernel
+    // clock samples it (one step per list period: the second, clear of the rebuild's tail).
+    struct StepKind {
+        int parity;
+        bool rebuild;
+        bool clocked;
+    };
+    StepKind next_kind() const {
+        const int iv = params.nlist_interval;
+        const int pick = iv > 1 ? 1 : 0;
+        StepKind k;
+        k.parity = cur;
+        k.rebuild = iv > 0 && since_rebuild + 1 >= iv;
+        k.clocked = kclock_on && (iv > 0 ? since_rebuild == pick : step % 8 == 0);
+        return k;
+    }
+    void enqueue_kind(const StepKind& k) {
+        kclock_step = k.clocked;
+        launch_step(k.parity);
+        kclock_step = false;   // steps launched directly (observed, adjoint) are never clocked
+        if (k.rebuild) launch_rebuild(k.parity ^ 1);
+    }
+    void advance(const StepKind& k) {
         cur ^= 1;
         step++;
         since_rebuild++;
-        if (params.nlist_interval > 0 && since_rebuild >= params.nlist_interval) {
-            launch_rebuild(cur);
+        if (k.rebuild) {
             since_rebuild = 0;
             n_rebuilds++;
         }
     }
+    // one step plus the rebuild it schedules
+    void enqueue_one() {
+        const StepKind k = next_kind();
+        enqueue_kind(k);
+        advance(k);
+    }
 
-    void build_graph() {
-        if (graph_exec) { HIP_TRY(hipGraphExecDestroy(graph_exec)); graph_exec = nullptr; }
-        const int n = cycle_steps();
-        // capture starting from parity 0 and a fresh rebuild cycle
-        const int save_cur = cur;
-        const long long save_step = step, save_since = since_rebuild, save_nr = n_rebuilds;
-        cur = 0;
-        since_rebuild = 0;
+    template <typename F> hipGraphExec_t capture(F&& enqueue) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < n; i++) enqueue_one();
+        enqueue();
         HIP_TRY(hipStreamEndCapture(stream, &g));
-        HIP_TRY(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+        hipGraphExec_t ex = nullptr;
+        HIP_TRY(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
         HIP_TRY(hipGraphDestroy(g));
+        return ex;
+    }
+
+    // Captures the cycle graph and every step kind that can occur, so that no later stepping
+    // call captures: replays of either kind can follow each other in any order.
+    void ensure_graphs() {
+        if (graphs_ready) return;
+        drop_graphs();
+        const int save_cur = cur;
+        const long long save_step = step, save_since = since_rebuild, save_nr = n_rebuilds, save_dev = dev_step;
+        const int iv = params.nlist_interval;
+        if (iv > 0) {
+            // the whole cycle, from parity 0 and a fresh rebuild cycle
+            const int n = cycle_steps();
+            cur = 0;
+            since_rebuild = 0;
+            graph_exec = capture([&] {
+                for (int i = 0; i < n; i++) enqueue_one();
+            });
+            graph_steps = n;
+        }
+        for (int p = 0; p < 2; p++)
+            for (int rb = 0; rb < (iv > 0 ? 2 : 1); rb++)
+                for (int ck = 0; ck < (kclock_on ? 2 : 1); ck++) {
+                    const StepKind k{p, rb != 0, ck != 0};
+                    step_graph[p][rb][ck] = capture([&] { enqueue_kind(k); });
+                }
         cur = save_cur;
         step = save_step;
         since_rebuild = save_since;
         n_rebuilds = save_nr;
-        graph_steps = n;
+        dev_step = save_dev;
+        graphs_ready = true;
     }
 
     // the device step counters follow the host's count (set before any stepping call enqueues)
     void sync_step_count() {
         set_step_count_kernel<<<(R + 63) / 64, 64, 0, stream>>>(step_count.p, R, step);
+        dev_step = step;
     }
 
     void enqueue_steps(int n) {
-        sync_step_count();
+        if (n <= 0) return;
+        if (dev_step != step) sync_step_count();
+        const bool graphs = cfg.use_graph != 0;
+        if (graphs) ensure_graphs();
         const int cyc = cycle_steps();
         while (n > 0) {
-            if (cfg.use_graph && params.nlist_interval > 0 && cur == 0 && since_rebuild == 0 && n >= cyc) {
-                if (!graph_exec) build_graph();
+            if (graphs && graph_exec && cur == 0 && since_rebuild == 0 && n >= cyc) {
                 HIP_TRY(hipGraphLaunch(graph_exec, stream));
                 step += cyc;
+                dev_step += cyc;
                 n_rebuilds += cyc / params.nlist_interval;
                 n -= cyc;
                 continue;
             }
-            enqueue_one();
+            const StepKind k = next_kind();
+            if (graphs) {
+                HIP_TRY(hipGraphLaunch(step_graph[k.parity][k.rebuild ? 1 : 0][k.clocked ? 1 : 0], stream));
+                dev_step++;
+            } else {
+                enqueue_kind(k);
+            }
+            advance(k);
             n--;
         }
     }
@@ -1377,11 +1457,47 @@ void get_parameters(lima_engine* E, int r, double* o) {
     if (ad) { o[66] = ad->k; o[67] = ad->t0; }
 }
 
+// Calls f(kb, b0) for every pair bond of system r (kind 0), or f(k_theta, theta_0) for every
+// angle (kind 1), compound terms first, then the bridges' in bridge order.
+template <typename F> void for_each_bonded(lima_engine* E, int r, int kind, F&& f) {
+    const ReplicaInfo& ri = E->reps_h[r];
+    if (kind == 0) {
+        for (int b = E->bond_off_h[ri.comp_begin]; b < E->bond_off_h[ri.comp_end]; b++) f(E->bonds_h[b].kb, E->bonds_h[b].b0);
+    } else {
+        for (int a = E->angle_off_h[ri.comp_begin]; a < E->angle_off_h[ri.comp_end]; a++) f(E->angles_h[a].k, E->angles_h[a].t0);
+    }
+    for (int b = 0; b < E->NB; b++) {
+        if (E->br_rep_h[b] != r) continue;
+        if (kind == 0) {
+            for (int t = E->brb_off_h[b]; t < E->brb_off_h[b + 1]; t++) f(E->brb_h[t].kb, E->brb_h[t].b0);
+        } else {
+            for (int t = E->bra_off_h[b]; t < E->bra_off_h[b + 1]; t++) f(E->bra_h[t].k, E->bra_h[t].t0);
+        }
+    }
+}
+
 void set_parameters(lima_engine* E, int r, const double* v) {
     for (int i = 0; i < LIMA_N_PARAMETERS; i++)
         if (!std::isfinite(v[i])) throw LimaError(LIMA_E_ARG, "parameters must be finite");
     for (int t = 0; t < kMaxTypes; t++)
         if (v[t] <= 0 || v[kMaxTypes + t] < 0) throw LimaError(LIMA_E_ARG, "sigma must be > 0 and epsilon >= 0");
+    // Entries 64-67 are one constant pair per kind shared by every term.  A system whose terms
+    // carry their own constants (an ingested molecule) keeps them when the entries are passed
+    // back unchanged, and refuses a change, which would replace the whole bonded force field.
+    double cur[LIMA_N_PARAMETERS];
+    get_parameters(E, r, cur);
+    bool write_kind[2] = {false, false};
+    for (int kind = 0; kind < 2; kind++) {
+        const float k_new = (float)v[64 + 2 * kind], x_new = (float)v[65 + 2 * kind];
+        if (k_new == (float)cur[64 + 2 * kind] && x_new == (float)cur[65 + 2 * kind]) continue;
+        bool uniform = true;
+        const float k0 = (float)cur[64 + 2 * kind], x0 = (float)cur[65 + 2 * kind];
+        for_each_bonded(E, r, kind, [&](float& k, float& x) { uniform = uniform && k == k0 && x == x0; });
+        if (!uniform)
+            throw LimaError(LIMA_E_UNSUPPORTED, kind == 0 ? "the system's pair bonds differ per term: kb, b0 (entries 64, 65) cannot be set"
+                                                          : "the system's angles differ per term: k_theta, theta_0 (entries 66, 67) cannot be set");
+        write_kind[kind] = true;
+    }
     HIP_TRY(hipSetDevice(E->device));
     HIP_TRY(hipStreamSynchronize(E->stream));
     lima_particle_params* pp = E->ff_h[r].particle_parameters;
@@ -1397,25 +1513,10 @@ void set_parameters(lima_engine* E, int r, const double* v) {
     HIP_TRY(hipMemcpy(E->tab_inter.p + (size_t)r * 1024, tinter.data(), 1024 * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(E->tab_intra.p + (size_t)r * 1024, tintra.data(), 1024 * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(E->tab_ss.p + r, tss.data(), 16, hipMemcpyHostToDevice));
-    const ReplicaInfo& ri = E->reps_h[r];
-    for (int b = E->bond_off_h[ri.comp_begin]; b < E->bond_off_h[ri.comp_end]; b++) {
-        E->bonds_h[b].kb = (float)v[64];
-        E->bonds_h[b].b0 = (float)v[65];
-    }
-    for (int a = E->angle_off_h[ri.comp_begin]; a < E->angle_off_h[ri.comp_end]; a++) {
-        E->angles_h[a].k = (float)v[66];
-        E->angles_h[a].t0 = (float)v[67];
-    }
-    for (int b = 0; b < E->NB; b++) {
-        if (E->br_rep_h[b] != r) continue;
-        for (int t = E->brb_off_h[b]; t < E->brb_off_h[b + 1]; t++) {
-            E->brb_h[t].kb = (float)v[64];
-            E->brb_h[t].b0 = (float)v[65];
-        }
-        for (int t = E->bra_off_h[b]; t < E->bra_off_h[b + 1]; t++) {
-            E->bra_h[t].k = (float)v[66];
-            E->bra_h[t].t0 = (float)v[67];
-        }
+    for (int kind = 0; kind < 2; kind++) {
+        if (!write_kind[kind]) continue;
+        const float k_new = (float)v[64 + 2 * kind], x_new = (float)v[65 + 2 * kind];
+        for_each_bonded(E, r, kind, [&](float& k, float& x) { k = k_new; x = x_new; });
     }
     auto put = [](auto& dbuf, const auto& host) {
         if (!host.empty()) HIP_TRY(hipMemcpy(dbuf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice));
@@ -1477,6 +1578,13 @@ void load_tune_file() {
         LaunchConfig c;
         if (!(is >> key >> c.spt >> c.chunk_particles >> c.listed_grid)) continue;
         if (!(is >> c.listed_lanes)) c.listed_lanes = 1;
+        // only shapes the autotuner itself can pick: any other line is skipped
+        const bool ok = (c.spt == 1 || c.spt == 2 || c.spt == 4) &&
+                        (c.chunk_particles == 256 || c.chunk_particles == 512 || c.chunk_particles == 1024 ||
+                         c.chunk_particles == 2048) && c.chunk_particles <= kChunkParticles &&
+                        (c.listed_grid == 1024 || c.listed_grid == 2048 || c.listed_grid == 4096 || c.listed_grid == 8192) &&
+                        (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes);
+        if (!ok) continue;
         g_tune_cache[key] = c;
     }
 }
@@ -1566,13 +1674,29 @@ int lima_step_async(lima_engine* E, int32_t n) {
     return guarded(E, [&] {
         if (n < 0) throw LimaError(LIMA_E_ARG, "n_steps < 0");
         HIP_TRY(hipSetDevice(E->device));
+        auto t0 = std::chrono::steady_clock::now();
+        if (!E->async_open && n > 0) {
+            // the device time of queued steps is read at the next lima_sync
+            HIP_TRY(hipEventRecord(E->ev_t0, E->stream));
+            E->async_open = true;
+        }
         E->enqueue_steps(n);
+        E->t_host_us += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
         return LIMA_OK;
     });
 }
 
 int lima_sync(lima_engine* E) {
     return guarded(E, [&] {
+        HIP_TRY(hipSetDevice(E->device));
+        if (E->async_open) {
+            HIP_TRY(hipEventRecord(E->ev_t1, E->stream));
+            HIP_TRY(hipEventSynchronize(E->ev_t1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, E->ev_t0, E->ev_t1));
+            E->t_device_us += ms * 1000.0;
+            E->async_open = false;
+        }
         HIP_TRY(hipStreamSynchronize(E->stream));
         return E->check_errors();
     });
@@ -1582,19 +1706,18 @@ int lima_step(lima_engine* E, int32_t n) {
     return guarded(E, [&] {
         if (n < 0) throw LimaError(LIMA_E_ARG, "n_steps < 0");
         HIP_TRY(hipSetDevice(E->device));
+        if (E->async_open) {   // steps queued by lima_step_async are timed up to here
+            int st = lima_sync(E);
+            if (st != LIMA_OK) return st;
+        }
         auto t0 = std::chrono::steady_clock::now();
-        hipEvent_t e0, e1;
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        HIP_TRY(hipEventRecord(e0, E->stream));
+        HIP_TRY(hipEventRecord(E->ev_t0, E->stream));
         E->enqueue_steps(n);
-        HIP_TRY(hipEventRecord(e1, E->stream));
+        HIP_TRY(hipEventRecord(E->ev_t1, E->stream));
         auto t1 = std::chrono::steady_clock::now();
-        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventSynchronize(E->ev_t1));
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        HIP_TRY(hipEventElapsedTime(&ms, E->ev_t0, E->ev_t1));
         E->t_device_us += ms * 1000.0;
         E->t_host_us += std::chrono::duration<double, std::micro>(t1 - t0).count();
         return E->check_errors();
@@ -1712,14 +1835,14 @@ int lima_step_logged(lima_engine* E, int32_t n, int32_t system, int32_t slots, f
         if (traindata && E->train_data.n < kLogRing * per_step) E->train_data.alloc(kLogRing * per_step);
         // the rings enter the captured step through its arguments: recapture with them, and again without
         auto reset_graph = [&] {
-            if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+            E->drop_graphs();
         };
         struct Off {   // logging is off again however the call ends
             lima_engine* E;
             ~Off() {
                 E->log_rep = -1;
                 E->log_train = false;
-                if (E->graph_exec) { (void)hipGraphExecDestroy(E->graph_exec); E->graph_exec = nullptr; }
+                E->drop_graphs();
             }
         } off{E};
         E->log_rep = system;
@@ -1996,7 +2119,7 @@ int lima_set_kernel_clock(lima_engine* E, int32_t on) {
         }
         if (want != E->kclock_on) {
             E->kclock_on = want;
-            if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+            E->drop_graphs();
         }
         return LIMA_OK;
     });
@@ -2069,7 +2192,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
             int timed = std::max(reps, cyc);
             // ms per step of the current shape: one cycle to capture and warm, then the timed steps
             auto trial = [&]() -> double {
-                if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+                E->drop_graphs();
                 E->build_scwork();
                 E->reps.upload(E->reps_h);
                 E->enqueue_steps(cyc);
@@ -2152,7 +2275,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
         E->cfg = best;
         E->build_scwork();
         E->reps.upload(E->reps_h);
-        if (E->graph_exec) { HIP_TRY(hipGraphExecDestroy(E->graph_exec)); E->graph_exec = nullptr; }
+        E->drop_graphs();
         return E->check_errors();
     });
 }

@@ -322,8 +322,11 @@ __global__ __launch_bounds__(256) void solvent_chain_kernel(StepArgs a) {
         }
 #pragma unroll
         for (int k = 0; k < SPT; k++) {
-            if (FIXED) { ix[k] += to_fixed(lx[k]); iy[k] += to_fixed(ly[k]); iz[k] += to_fixed(lz[k]); }
-            else { fx[k] += lx[k]; fy[k] += ly[k]; fz[k] += lz[k]; }
+            if (FIXED) {
+                uint32_t bad = 0;
+                ix[k] += to_fixed(lx[k], bad); iy[k] += to_fixed(ly[k], bad); iz[k] += to_fixed(lz[k], bad);
+                if (bad) atomicOr(a.err, bad);
+            } else { fx[k] += lx[k]; fy[k] += ly[k]; fz[k] += lz[k]; }
         }
     }
     double esum = 0, vsum = 0;
@@ -509,15 +512,21 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             if (hs == 0 && active) {
                 const float4 o = xch[pi];
                 const size_t slot = (size_t)w * kCompoundBlock + pi;
-                if (FIXED)
-                    a.partC_fx[slot] = I3{to_fixed(lx) + to_fixed(o.x), to_fixed(ly) + to_fixed(o.y), to_fixed(lz) + to_fixed(o.z)};
-                else
+                if (FIXED) {
+                    uint32_t bad = 0;
+                    a.partC_fx[slot] = I3{to_fixed(lx, bad) + to_fixed(o.x, bad), to_fixed(ly, bad) + to_fixed(o.y, bad),
+                                          to_fixed(lz, bad) + to_fixed(o.z, bad)};
+                    if (bad) atomicOr(a.err, bad);
+                } else
                     a.partC[slot] = make_float4(lx + o.x, ly + o.y, lz + o.z, 0.f);
             }
         } else if (active) {
             const size_t slot = (size_t)w * kCompoundBlock + p;
-            if (FIXED) a.partC_fx[slot] = I3{to_fixed(lx), to_fixed(ly), to_fixed(lz)};
-            else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
+            if (FIXED) {
+                uint32_t bad = 0;
+                a.partC_fx[slot] = I3{to_fixed(lx, bad), to_fixed(ly, bad), to_fixed(lz, bad)};
+                if (bad) atomicOr(a.err, bad);
+            } else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
         }
     }
 }
@@ -592,10 +601,11 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
         }
     }
     long long fx_i = 0, fy_i = 0, fz_i = 0;
+    uint32_t bad = 0;
     if (FIXED) {
-        fx_i = to_fixed(force.x) + to_fixed(bs.x) + to_fixed(as.x) + (a.enable_dihedrals ? to_fixed(ds.x) : 0);
-        fy_i = to_fixed(force.y) + to_fixed(bs.y) + to_fixed(as.y) + (a.enable_dihedrals ? to_fixed(ds.y) : 0);
-        fz_i = to_fixed(force.z) + to_fixed(bs.z) + to_fixed(as.z) + (a.enable_dihedrals ? to_fixed(ds.z) : 0);
+        fx_i = to_fixed(force.x, bad) + to_fixed(bs.x, bad) + to_fixed(as.x, bad) + (a.enable_dihedrals ? to_fixed(ds.x, bad) : 0);
+        fy_i = to_fixed(force.y, bad) + to_fixed(bs.y, bad) + to_fixed(as.y, bad) + (a.enable_dihedrals ? to_fixed(ds.y, bad) : 0);
+        fz_i = to_fixed(force.z, bad) + to_fixed(bs.z, bad) + to_fixed(as.z, bad) + (a.enable_dihedrals ? to_fixed(ds.z, bad) : 0);
     } else {
         force = f3_add(force, bs);
         force = f3_add(force, as);
@@ -628,8 +638,10 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
             }
         }
     }
-    if (FIXED) { fx_i += to_fixed(lx); fy_i += to_fixed(ly); fz_i += to_fixed(lz); }
-    else force = f3_add(force, f3(lx, ly, lz));
+    if (FIXED) {
+        fx_i += to_fixed(lx, bad); fy_i += to_fixed(ly, bad); fz_i += to_fixed(lz, bad);
+        if (bad) atomicOr(a.err, bad);
+    } else force = f3_add(force, f3(lx, ly, lz));
     // the sum so far, in the order above; compound_integrate_kernel adds the listed partials to it
     if (active) {
         if (FIXED) a.cloc_fx[gi] = I3{fx_i, fy_i, fz_i};
@@ -1041,7 +1053,9 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
     }
     const float4 l = a.ss_part[si];
     if (FIXED) {
-        ix += to_fixed(l.x); iy += to_fixed(l.y); iz += to_fixed(l.z);
+        uint32_t bad = 0;
+        ix += to_fixed(l.x, bad); iy += to_fixed(l.y, bad); iz += to_fixed(l.z, bad);
+        if (bad) atomicOr(a.err, bad);
         fx = from_fixed(ix); fy = from_fixed(iy); fz = from_fixed(iz);
     } else {
         fx += l.x; fy += l.y; fz += l.z;

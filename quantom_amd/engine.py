@@ -2,11 +2,12 @@
 
 `Engine` keeps the reference's step API (Engine.cuh:109-208):
 `Engine(simulation, params)` builds the device state and the first neighbour
-lists (Engine.cu:9-38); `deviceMaster()` runs one step (Engine.cu:44-47,
-400-451); `hostMaster()` is the reference's per-step host hook (Engine.cu:49-75),
-whose work (thermostat, neighbour-list rebuild) now runs on the device inside
-the step, so here it only surfaces errors.  A `Simulation` holds one or more
-systems (replicas) stepped together.
+lists (Engine.cu:9-38); `deviceMaster()` queues one step (Engine.cu:44-47,
+400-451) as a captured-graph replay; `hostMaster()` is the reference's per-step
+host hook (Engine.cu:49-75), whose work (thermostat, neighbour-list rebuild) now
+runs on the device inside the step, so here it waits for the device once per
+list period and surfaces errors.  A `Simulation` holds one or more systems
+(replicas) stepped together.
 
 The native library must be present: there is no fallback path.
 """
@@ -150,12 +151,15 @@ class Engine:
 
     # ---- the reference's step API ----
     def deviceMaster(self):
-        """One step on the device (Engine::deviceMaster, Engine.cu:44-47)."""
-        self.step(1)
+        """Queues one step on the device (Engine::deviceMaster, Engine.cu:44-47)."""
+        self.step_async(1)
+        self._queued = getattr(self, "_queued", 0) + 1
 
     def hostMaster(self):
-        """Per-step host hook (Engine::hostMaster, Engine.cu:49-75); the work runs on the device."""
-        self.sync()
+        """Per-step host hook (Engine::hostMaster, Engine.cu:49-75).  Its work runs on the device
+        inside the step; this waits once per list period and raises device errors."""
+        if getattr(self, "_queued", 0) >= max(1, self.params.nlist_interval):
+            self.sync()
 
     def step(self, n: int = 1):
         self._check(self.L.lima_step(self.h, n))
@@ -166,6 +170,7 @@ class Engine:
         self.steps += n
 
     def sync(self):
+        self._queued = 0
         self._check(self.L.lima_sync(self.h))
 
     def rebuild_neighbors(self):
@@ -179,9 +184,21 @@ class Engine:
         self._check(self.L.lima_get_state(self.h, _f(cx), _f(cxm), _f(sx), _f(sxm)))
         return cx, cxm, sx, sxm
 
-    def set_state(self, cx, cxm, sx, sxm):
-        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (cx, cxm, sx, sxm)]
-        self._check(self.L.lima_set_state(self.h, *[_f(v) for v in a]))
+    def set_state(self, cx=None, cxm=None, sx=None, sxm=None):
+        """Overwrites the given arrays of the state; None keeps that array (NULL in the C call)."""
+        sim = self.simulation
+        shapes = ((sim.n_chain, 3), (sim.n_chain, 3), (sim.n_solvents, 3), (sim.n_solvents, 3))
+        args = []
+        for v, shape in zip((cx, cxm, sx, sxm), shapes):
+            if v is None:
+                args.append(None)
+                continue
+            v = np.ascontiguousarray(v, dtype=np.float32)
+            if v.shape != shape:
+                raise ValueError(f"state array of shape {v.shape}, expected {shape}")
+            args.append(v)
+        self._keep_state = args     # alive for the duration of the call
+        self._check(self.L.lima_set_state(self.h, *[_f(v) for v in args]))
 
     def forces(self):
         sim = self.simulation

@@ -89,3 +89,33 @@ def test_t4lysozyme_trajectory(accum):
     eo = o32.energies()
     assert e["temperature"] == pytest.approx(eo["temperature"], rel=1e-3)
     eng.close()
+
+
+def test_parameter_round_trip_keeps_per_term_bonded_constants():
+    """The demo's bonds and angles carry their own constants (ingested per term).  Passing the
+    parameter vector back unchanged keeps them: the forces stay bit-equal.  Changing a shared
+    bonded entry is refused, since it would replace every term's constant."""
+    from quantom_amd.abi import LIMA_E_UNSUPPORTED
+    from quantom_amd.engine import Engine, LimaStatusError
+    print("case: demo get/set parameters round trip", flush=True)
+    s = _system()
+    assert len(np.unique(s.bonds["kb"])) > 1 and len(np.unique(s.angles["k_theta"])) > 1
+    eng = Engine([s], _params(nlist_interval=0))
+    f0 = eng.compute_forces()
+    v = eng.get_parameters(0)
+    eng.set_parameters(v, 0)
+    f1 = eng.compute_forces()
+    np.testing.assert_array_equal(f0[0], f1[0])
+    np.testing.assert_array_equal(f0[1], f1[1])
+    for idx in (64, 66):
+        w = v.copy()
+        w[idx] *= 1.01
+        with pytest.raises(LimaStatusError) as ex:
+            eng.set_parameters(w, 0)
+        assert ex.value.code == LIMA_E_UNSUPPORTED
+    # the LJ entries of the same vector may still change
+    w = v.copy()
+    w[33] *= 1.01
+    eng.set_parameters(w, 0)
+    f2 = eng.compute_forces()
+    assert np.abs(f2[0] - f0[0]).max() > 0
