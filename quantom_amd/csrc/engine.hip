@@ -1479,8 +1479,13 @@ template <typename F> void for_each_bonded(lima_engine* E, int r, int kind, F&& 
 void set_parameters(lima_engine* E, int r, const double* v) {
     for (int i = 0; i < LIMA_N_PARAMETERS; i++)
         if (!std::isfinite(v[i])) throw LimaError(LIMA_E_ARG, "parameters must be finite");
-    for (int t = 0; t < kMaxTypes; t++)
-        if (v[t] <= 0 || v[kMaxTypes + t] < 0) throw LimaError(LIMA_E_ARG, "sigma must be > 0 and epsilon >= 0");
+    // a type slot the force field leaves empty (sigma 0) may be passed back as it is; a new value
+    // must be a valid one
+    const lima_particle_params* pp_cur = E->ff_h[r].particle_parameters;
+    for (int t = 0; t < kMaxTypes; t++) {
+        const bool same = (float)v[t] == pp_cur[t].sigma && (float)v[kMaxTypes + t] == pp_cur[t].epsilon;
+        if (!same && (v[t] <= 0 || v[kMaxTypes + t] < 0)) throw LimaError(LIMA_E_ARG, "sigma must be > 0 and epsilon >= 0");
+    }
     // Entries 64-67 are one constant pair per kind shared by every term.  A system whose terms
     // carry their own constants (an ingested molecule) keeps them when the entries are passed
     // back unchanged, and refuses a change, which would replace the whole bonded force field.
