@@ -198,6 +198,16 @@ int lima_get_energies(lima_engine* engine, lima_energies* out);
    step's displacement (what the thermostat uses).  out: [ceil(n_steps/every)][n_systems].
    One host synchronisation at the end. */
 int lima_step_observed(lima_engine* engine, int32_t n_steps, int32_t every, lima_energies* out);
+/* No counterpart in the reference: for callers that cannot allocate device memory
+   (tight serving loops, code that forbids hipMalloc).  lima_step_observed with its
+   energy series staged in a fixed device array of the library (4 MiB per device,
+   84 bytes per record and system: ceil(n_steps/every) x n_systems records at most
+   49,000 or so) instead of a per-call allocation, and with the plain steps between
+   records replayed from launch graphs already built (or launched directly), never
+   captured.  Calls no hipMalloc/hipFree.  Larger requests return LIMA_E_ARG.  The
+   array is shared by every engine on a device: calls that overlap on one device
+   (from several host threads) race.  Results equal lima_step_observed's. */
+int lima_step_observed_noalloc(lima_engine* engine, int32_t n_steps, int32_t every, lima_energies* out);
 /* lima_step with the per-step logs the reference keeps for one system
    (Engine::offloadLoggingData / offloadTrainData, Engine.cu:288-310; the
    compoundKernel's data log, Engine.cu:1019-1046), copied out every 100 steps.

@@ -27,6 +27,15 @@
 
 using namespace lima;
 
+// Staging of lima_step_observed_noalloc (an energy series, [records][systems][kEnergyTerms]
+// doubles then [records][systems] floats of lambda): a file-scope device array, one copy per
+// device, held until the process ends, so that the entry point calls no allocator.  In host
+// code `scratch` only names the symbol; its device address comes from hipGetSymbolAddress,
+// looked up once per device at engine creation (scratch_address).  Every call on a device
+// shares its copy: calls that overlap on one device race.
+constexpr size_t kObservedScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kObservedScratchBytes];
+
 namespace {
 
 #define HIP_TRY(expr)                                                                         \
@@ -68,6 +77,24 @@ template <typename T> struct DBuf {
         n = 0;
     }
 };
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+// the device address of `scratch` on the current device `device`; null past kScratchDevices
+unsigned char* scratch_address(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !ptr)
+            throw LimaError(LIMA_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+        g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
 
 struct LaunchConfig {
     int spt = 2;                  // solvents per thread in solvent_chain_kernel (1, 2, 4)
@@ -183,6 +210,7 @@ struct lima_engine {
     int graph_steps = 0;
     long long dev_step = -1;      // the device step counters' value after the queued work (-1: unknown)
     bool async_open = false;      // ev_t0 marks the start of steps queued by lima_step_async
+    unsigned char* scratch_d = nullptr;   // this device's copy of `scratch` (no-alloc staging)
 
     // ---- run state ----
     int cur = 0;                  // ping-pong parity of the current state
@@ -558,10 +586,11 @@ struct lima_engine {
         dev_step = step;
     }
 
-    void enqueue_steps(int n) {
+    // allow_capture false: replays only graphs already built, else launches directly
+    void enqueue_steps(int n, bool allow_capture = true) {
         if (n <= 0) return;
         if (dev_step != step) sync_step_count();
-        const bool graphs = cfg.use_graph != 0;
+        const bool graphs = cfg.use_graph != 0 && (allow_capture || graphs_ready);
         if (graphs) ensure_graphs();
         const int cyc = cycle_steps();
         while (n > 0) {
@@ -1622,6 +1651,40 @@ void get3(const std::vector<float4>& v, float* out) {
     for (size_t i = 0; i < v.size(); i++) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
 }
 
+// The steps of lima_step_observed: record steps launched directly with their energy sums, each
+// record's sums and lambda copied into series [record][system][kEnergyTerms] and lser
+// [record][system] (every slot of a record is written before it is read), the plain steps
+// between records as graph replays.  allow_capture false keeps to launches and replays of
+// graphs already built (no capture or instantiation).
+void observed_steps(lima_engine* E, int n, int every, double* series, float* lser, bool allow_capture) {
+    const int R = E->R;
+    int rec = 0;
+    E->sync_step_count();
+    for (int s = 0; s < n;) {
+        // the recorded step: the step itself with its energy sums (Engine::step + the Analyzer's sums)
+        HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
+        E->launch_step(E->cur, true);
+        HIP_TRY(hipMemcpyAsync(series + (size_t)rec * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8,
+                               hipMemcpyDeviceToDevice, E->stream));
+        HIP_TRY(hipMemcpyAsync(lser + (size_t)rec * R, E->lambda_applied.p, (size_t)R * 4, hipMemcpyDeviceToDevice,
+                               E->stream));
+        E->cur ^= 1;
+        E->step++;
+        E->since_rebuild++;
+        if (E->params.nlist_interval > 0 && E->since_rebuild >= E->params.nlist_interval) {
+            E->launch_rebuild(E->cur);
+            E->since_rebuild = 0;
+            E->n_rebuilds++;
+        }
+        rec++;
+        s++;
+        // the plain steps up to the next record (graph replays where the cycle aligns)
+        const int run = std::min(every - 1, n - s);
+        if (run > 0) E->enqueue_steps(run, allow_capture);
+        s += run;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1646,6 +1709,7 @@ int lima_create(const lima_system* systems, int32_t n_systems, const lima_run_pa
         if (params->device < 0 || params->device >= ndev) throw LimaError(LIMA_E_ARG, "device ordinal out of range");
         E->device = params->device;
         HIP_TRY(hipSetDevice(E->device));
+        E->scratch_d = scratch_address(E->device);
         E->create_streams();
         validate_and_build(E, systems, n_systems);
         // Engine::Engine -> handleNLISTS(force) (Engine.cu:35)
@@ -1889,31 +1953,7 @@ int lima_step_observed(lima_engine* E, int32_t n, int32_t every, lima_energies* 
         DBuf<float> lser;
         series.alloc(e.size());
         lser.alloc(lam.size());
-        int rec = 0;
-        E->sync_step_count();
-        for (int s = 0; s < n;) {
-            // the recorded step: the step itself with its energy sums (Engine::step + the Analyzer's sums)
-            HIP_TRY(hipMemsetAsync(E->energy.p, 0, E->energy.n * sizeof(double), E->stream));
-            E->launch_step(E->cur, true);
-            HIP_TRY(hipMemcpyAsync(series.p + (size_t)rec * R * kEnergyTerms, E->energy.p, (size_t)R * kEnergyTerms * 8,
-                                   hipMemcpyDeviceToDevice, E->stream));
-            HIP_TRY(hipMemcpyAsync(lser.p + (size_t)rec * R, E->lambda_applied.p, (size_t)R * 4, hipMemcpyDeviceToDevice,
-                                   E->stream));
-            E->cur ^= 1;
-            E->step++;
-            E->since_rebuild++;
-            if (E->params.nlist_interval > 0 && E->since_rebuild >= E->params.nlist_interval) {
-                E->launch_rebuild(E->cur);
-                E->since_rebuild = 0;
-                E->n_rebuilds++;
-            }
-            rec++;
-            s++;
-            // the plain steps up to the next record (graph replays where the cycle aligns)
-            const int run = std::min(every - 1, n - s);
-            if (run > 0) E->enqueue_steps(run);
-            s += run;
-        }
+        observed_steps(E, n, every, series.p, lser.p, true);
         HIP_TRY(hipStreamSynchronize(E->stream));
         if (nrec > 0) {
             HIP_TRY(hipMemcpy(e.data(), series.p, (size_t)nrec * R * kEnergyTerms * 8, hipMemcpyDeviceToHost));
@@ -1922,6 +1962,42 @@ int lima_step_observed(lima_engine* E, int32_t n, int32_t every, lima_energies* 
         for (int k = 0; k < nrec; k++)
             for (int r = 0; r < R; r++)
                 fill_energies(E, r, &e[((size_t)k * R + r) * kEnergyTerms], lam[(size_t)k * R + r], out[(size_t)k * R + r]);
+        return E->check_errors();
+    });
+}
+
+int lima_step_observed_noalloc(lima_engine* E, int32_t n, int32_t every, lima_energies* out) {
+    return guarded(E, [&] {
+        if (n < 0 || every <= 0) throw LimaError(LIMA_E_ARG, "n_steps < 0 or every <= 0");
+        if (!out && n > 0) throw LimaError(LIMA_E_ARG, "null output");
+        HIP_TRY(hipSetDevice(E->device));
+        const int R = E->R;
+        const size_t nrec = (size_t)((n + every - 1) / every);
+        const size_t ser_bytes = nrec * R * kEnergyTerms * sizeof(double);
+        const size_t lam_off = (ser_bytes + 255) & ~(size_t)255;
+        if (lam_off + nrec * R * sizeof(float) > kObservedScratchBytes) {
+            char buf[200];
+            snprintf(buf, sizeof(buf), "%zu records x %d systems exceed the no-alloc staging of %zu bytes (%zu bytes per record and system)",
+                     nrec, R, kObservedScratchBytes, kEnergyTerms * sizeof(double) + sizeof(float));
+            throw LimaError(LIMA_E_ARG, buf);
+        }
+        unsigned char* base = E->scratch_d;
+        if (!base) throw LimaError(LIMA_E_UNSUPPORTED, "no-alloc staging is not set up on this device");
+        double* series = reinterpret_cast<double*>(base);
+        float* lser = reinterpret_cast<float*>(base + lam_off);
+        observed_steps(E, n, every, series, lser, false);
+        HIP_TRY(hipStreamSynchronize(E->stream));
+        // out of the staging in fixed host chunks (no host allocation either)
+        constexpr size_t kChunk = 64;
+        double e[kChunk * kEnergyTerms];
+        float lam[kChunk];
+        const size_t total = nrec * R;
+        for (size_t i = 0; i < total; i += kChunk) {
+            const size_t m = std::min(kChunk, total - i);
+            HIP_TRY(hipMemcpy(e, series + i * kEnergyTerms, m * kEnergyTerms * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(lam, lser + i, m * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t j = 0; j < m; j++) fill_energies(E, (int)((i + j) % R), &e[j * kEnergyTerms], lam[j], out[i + j]);
+        }
         return E->check_errors();
     });
 }

@@ -31,7 +31,7 @@ EXPORTS = (
     "lima_get_forces", "lima_compute_forces", "lima_get_energies", "lima_get_neighbor_counts",
     "lima_get_neighbors", "lima_set_neighbors", "lima_get_timings", "lima_time_kernels", "lima_set_kernel_clock", "lima_get_kernel_clock", "lima_get_config",
     "lima_autotune", "lima_get_parameters", "lima_set_parameters", "lima_adjoint_run", "lima_record_frames",
-    "lima_step_observed", "lima_step_logged",
+    "lima_step_observed", "lima_step_observed_noalloc", "lima_step_logged",
 )
 
 
@@ -81,6 +81,7 @@ def load_library() -> C.CDLL:
     L.lima_adjoint_run.argtypes = [vp, C.POINTER(CAdjointSpec), dp, dp]
     L.lima_record_frames.argtypes = [vp, C.c_int32, fp, fp, dp]
     L.lima_step_observed.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(CEnergies)]
+    L.lima_step_observed_noalloc.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(CEnergies)]
     L.lima_step_logged.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, fp, fp]
     _lib = L
     return L
@@ -218,13 +219,15 @@ class Engine:
         self._check(self.L.lima_get_energies(self.h, arr))
         return [arr[k].as_dict() for k in range(n)]
 
-    def step_observed(self, n: int, every: int = 1) -> list:
+    def step_observed(self, n: int, every: int = 1, noalloc: bool = False) -> list:
         """n steps; the first and every `every`-th step after it record each system's energies.
-        Returns [record][system] dicts (lima_energies fields)."""
+        Returns [record][system] dicts (lima_energies fields).  noalloc: the entry point that
+        stages the series in the library's fixed device array (no device allocation)."""
         R = len(self.simulation.systems)
         nrec = (n + every - 1) // every
         arr = (CEnergies * max(nrec * R, 1))()
-        self._check(self.L.lima_step_observed(self.h, n, every, arr))
+        fn = self.L.lima_step_observed_noalloc if noalloc else self.L.lima_step_observed
+        self._check(fn(self.h, n, every, arr))
         self.steps += n
         return [[arr[k * R + r].as_dict() for r in range(R)] for k in range(nrec)]
 
