@@ -21,7 +21,8 @@ KIND_CC, KIND_CS, KIND_SC, KIND_SS = 0, 1, 2, 3
 def lib(fast: bool = False) -> C.CDLL:
     name = "liblima_oracle_fast.so" if fast else "liblima_oracle.so"
     if name not in _libs:
-        path = os.path.join(_HERE, name)
+        # LIMA_ORACLE_DIR: another build of the same source (the sanitizer test's ASan/UBSan build)
+        path = os.path.join(os.environ.get("LIMA_ORACLE_DIR", _HERE), name)
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
         L = C.CDLL(path)

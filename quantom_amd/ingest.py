@@ -53,9 +53,11 @@ class CIngestInfo(C.Structure):
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise FileNotFoundError(f"{LIB_PATH} missing: run `python -m quantom_amd.build`")
-        L = C.CDLL(LIB_PATH)
+        # LIMA_INGEST_LIB: another build of the same source (the sanitizer test's ASan/UBSan build)
+        path = os.environ.get("LIMA_INGEST_LIB", LIB_PATH)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `python -m quantom_amd.build`")
+        L = C.CDLL(path)
         L.lima_ff_merge.argtypes = [C.POINTER(C.c_char_p), C.c_int32, C.c_char_p]
         L.lima_ff_prepare.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
         L.lima_ingest_default_params.argtypes = [C.POINTER(CIngestParams)]
