@@ -33,19 +33,21 @@ def log(*a):
 def workload(name):
     from quantom_amd import systems
     if name == "production":
-        return [systems.production()], "production: 10 chains x 10 compounds x 128 + lattice solvent 33/nm^3, 10 nm, 16 types"
+        return [systems.production()], ("production: 10 chains x 10 compounds x 128 + lattice solvent 33/nm^3, 10 nm, "
+                                        "16 types (compounds built within 1.6 nm, central numbering, sphere margin 1.25 nm)")
     if name == "small":
         return [systems.small()], "small: 1 chain x 2 compounds x 128 + 3000 solvent, 5 nm, 8 types"
     if name == "large":
-        return [systems.large()], "large: 10 chains x 40 compounds x 128 + lattice solvent 33/nm^3, 18 nm, 16 types"
+        return [systems.large()], ("large: 10 chains x 40 compounds x 128 + lattice solvent 33/nm^3, 18 nm, 16 types "
+                                   "(compounds built within 1.6 nm, central numbering, sphere margin 1.25 nm)")
     if name == "t4lysozyme":
         from quantom_amd.abi import LimaSystem
         s = LimaSystem.load(os.path.join(ROOT, "tests", "golden", "t4lysozyme.npz"))
         return [s], ("t4lysozyme: the reference's demo (Demo/6lzm) as ingested: 36 compounds, 1293 heavy atoms, "
                      "5000 waters, 7 nm, 19 types, dt 0.1 fs")
     if name == "replicas":
-        return systems.replicas(256), ("replicas: 256 systems, chain 128-1024 in compounds of 64, solvent 500-6000, "
-                                       "5.5-6 nm")
+        return systems.replicas(256), ("replicas: 256 systems, chain 128-1024 in compounds of 16 (box < 4.6 nm) "
+                                       "or 32, solvent 500-6000, box edge at 33/nm^3 within 3-6 nm, T0 280-320 K")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -189,6 +191,47 @@ def accuracy_report(args):
         out[f"production_rms_force_error_{name}"] = rel_rms(np.concatenate([cf, sf]), ref)
         e.close()
     log(f"force errors: {out}")
+
+    def slope(t_ns, s, n_part):
+        return float(np.polyfit(t_ns, np.asarray(s) / n_part, 1)[0])
+
+    # production energy drift: equilibrated with the thermostat on the GPU, then NVE from that
+    # same state on the GPU (fp32, fixed64) and in the FP64 oracle (a shorter window)
+    eq = Engine([prod], RunParams(nlist_interval=10))
+    eq.step(args.drift_equilibrate)
+    cx, cxm, sx, sxm = eq.get_state()
+    eq.close()
+    ps = prod.copy()
+    ps.chain_positions, ps.chain_prev_positions = cx.copy(), cxm.copy()
+    ps.solvent_positions, ps.solvent_prev_positions = sx.copy(), sxm.copy()
+    n_part = ps.n_chain + ps.n_solvents
+    every = 100
+    prod_series = {}
+    for name, acc in (("fp32", ACCUM_FP32), ("fixed64", ACCUM_FIXED64)):
+        e = Engine([ps], RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE, accumulation=acc))
+        recs = e.step_observed(args.drift_steps_production, every)
+        prod_series[name] = [r[0]["potential"] + r[0]["kinetic"] for r in recs]
+        e.close()
+    o_every = 50
+    o = Oracle(ps, RunParams(nlist_interval=10, thermostat=THERMOSTAT_NONE), precision=64, fast=True)
+    es = []
+    for k in range(0, args.drift_steps_oracle, o_every):
+        en = o.energies()
+        es.append(en["potential"] + en["kinetic"])
+        o.step(o_every)
+    t_o = np.arange(len(es)) * o_every * ps.dt
+    out["production_energy_drift_fp64_oracle_J_per_mol_per_particle_per_ns"] = slope(t_o, es, n_part)
+    for name, ser in prod_series.items():
+        t_g = np.arange(len(ser)) * every * ps.dt
+        out[f"production_energy_drift_{name}_J_per_mol_per_particle_per_ns"] = slope(t_g, ser, n_part)
+        m = int(np.searchsorted(t_g, t_o[-1], side="right"))
+        out[f"production_energy_drift_{name}_oracle_window_J_per_mol_per_particle_per_ns"] = slope(t_g[:m], ser[:m], n_part)
+    out["production_drift"] = {"equilibration_steps": args.drift_equilibrate, "gpu_steps": args.drift_steps_production,
+                               "oracle_steps": args.drift_steps_oracle, "record_every_gpu": every,
+                               "record_every_oracle": o_every, "mean_energy_J_per_mol_per_particle":
+                               float(np.mean(prod_series["fp32"]) / n_part)}
+    log(f"production drift: {out}")
+
     # energy drift: NVE, small system, GPU (fp32, fixed64) and the FP64 oracle
     small = systems.small()
     steps, every = args.drift_steps, 50
@@ -215,9 +258,8 @@ def accuracy_report(args):
     series["fp64_oracle"] = es
     t_ns = np.arange(len(es)) * every * small.dt
     n_part = small.n_chain + small.n_solvents
-    for name, s in series.items():
-        slope = float(np.polyfit(t_ns, np.asarray(s) / n_part, 1)[0])
-        out[f"small_energy_drift_{name}_J_per_mol_per_particle_per_ns"] = slope
+    for name, ser in series.items():
+        out[f"small_energy_drift_{name}_J_per_mol_per_particle_per_ns"] = slope(t_ns, ser, n_part)
     out["drift_window_steps"] = steps
     out["note"] = ("angle and dihedral forces of the reference are not gradients of their energies, "
                    "so the drift is compared with the FP64 oracle's own drift")
@@ -346,6 +388,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--drift-steps", type=int, default=1000)
+    ap.add_argument("--drift-steps-production", type=int, default=10000)
+    ap.add_argument("--drift-steps-oracle", type=int, default=500)
+    ap.add_argument("--drift-equilibrate", type=int, default=2000)
     ap.add_argument("--sequential", action="store_true",
                     help="replicas workload: also time the replicas as sequential single-system engines")
     ap.add_argument("--no-autotune", action="store_true",

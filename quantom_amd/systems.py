@@ -64,10 +64,13 @@ class _Grid:
         self.L = L
         self.n = max(1, int(L // h))
         self.cells: dict = {}
+        self._nbr: dict = {}
 
     def key(self, p):
-        q = np.floor(np.mod(p, self.L) / self.L * self.n).astype(int) % self.n
-        return (int(q[0]), int(q[1]), int(q[2]))
+        # any consistent cell of edge >= h serves: the 3x3x3 block then holds every bead within h
+        L, n = self.L, self.n
+        return (int(math.floor((float(p[0]) % L) / L * n)) % n, int(math.floor((float(p[1]) % L) / L * n)) % n,
+                int(math.floor((float(p[2]) % L) / L * n)) % n)
 
     def add(self, idx, p):
         self.cells.setdefault(self.key(p), []).append(idx)
@@ -75,35 +78,75 @@ class _Grid:
     def remove(self, idx, p):
         self.cells[self.key(p)].remove(idx)
 
+    def _around(self, key):
+        """The distinct cells of the 3x3x3 block around a cell, in a fixed order (cached)."""
+        got = self._nbr.get(key)
+        if got is None:
+            cx, cy, cz = key
+            rng = (-1, 0, 1) if self.n >= 3 else tuple(range(self.n))
+            got = []
+            for dx in rng:
+                for dy in rng:
+                    for dz in rng:
+                        k = ((cx + dx) % self.n, (cy + dy) % self.n, (cz + dz) % self.n)
+                        if k not in got:
+                            got.append(k)
+            self._nbr[key] = got
+        return got
+
     def near(self, p):
-        cx, cy, cz = self.key(p)
         out = []
-        rng = (-1, 0, 1) if self.n >= 3 else tuple(range(self.n))
-        seen = set()
-        for dx in rng:
-            for dy in rng:
-                for dz in rng:
-                    k = ((cx + dx) % self.n, (cy + dy) % self.n, (cz + dz) % self.n)
-                    if k in seen:
-                        continue
-                    seen.add(k)
-                    out.extend(self.cells.get(k, ()))
+        cells = self.cells
+        for k in self._around(self.key(p)):
+            c = cells.get(k)
+            if c:
+                out.extend(c)
         return out
+
+
+def _cross(a, b):
+    """a x b of two 3-vectors, each component a_i*b_j - a_j*b_i as np.cross rounds it."""
+    a0, a1, a2 = float(a[0]), float(a[1]), float(a[2])
+    b0, b1, b2 = float(b[0]), float(b[1]), float(b[2])
+    return np.array([a1 * b2 - a2 * b1, a2 * b0 - a0 * b2, a0 * b1 - a1 * b0])
+
+
+def _frame(a, b, c):
+    """The NeRF frame (bc, m, n) at c for the chain a-b-c."""
+    bc = c - b
+    bc /= np.linalg.norm(bc)
+    n = _cross(b - a, bc)
+    nn = np.linalg.norm(n)
+    if nn < 1e-9:
+        n = _cross(bc, np.array([1.0, 0.0, 0.0]))
+        if np.linalg.norm(n) < 1e-9:
+            n = _cross(bc, np.array([0.0, 1.0, 0.0]))
+        nn = np.linalg.norm(n)
+    n /= nn
+    return bc, _cross(n, bc), n
+
+
+def _place_in(c, frame, bond, angle, torsion):
+    """_place with the frame of (a, b, c) given."""
+    bc, m, n = frame
+    d2 = np.array([-bond * math.cos(angle), bond * math.sin(angle) * math.cos(torsion),
+                   bond * math.sin(angle) * math.sin(torsion)])
+    return c + d2[0] * bc + d2[1] * m + d2[2] * n
 
 
 def _place(a, b, c, bond, angle, torsion):
     """NeRF: position of d given a, b, c, |cd|, angle bcd and torsion abcd."""
     bc = c - b
     bc /= np.linalg.norm(bc)
-    n = np.cross(b - a, bc)
+    n = _cross(b - a, bc)
     nn = np.linalg.norm(n)
     if nn < 1e-9:
-        n = np.cross(bc, np.array([1.0, 0.0, 0.0]))
+        n = _cross(bc, np.array([1.0, 0.0, 0.0]))
         if np.linalg.norm(n) < 1e-9:
-            n = np.cross(bc, np.array([0.0, 1.0, 0.0]))
+            n = _cross(bc, np.array([0.0, 1.0, 0.0]))
         nn = np.linalg.norm(n)
     n /= nn
-    m = np.cross(n, bc)
+    m = _cross(n, bc)
     d2 = np.array([-bond * math.cos(angle), bond * math.sin(angle) * math.cos(torsion),
                    bond * math.sin(angle) * math.sin(torsion)])
     return c + d2[0] * bc + d2[1] * m + d2[2] * n
@@ -120,9 +163,16 @@ def _walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size):
     grid = _Grid(L, EXCLUSION_DIST)
     excl2 = EXCLUSION_DIST ** 2
 
+    conf2_lo = conf2_hi = 0.0
+    if confine is not None:
+        conf2_lo, conf2_hi = confine * confine * (1 - 1e-9), confine * confine * (1 + 1e-9)
+
     def ok(p, gidx, chain_start, comp_first):
         if confine is not None and comp_first is not None:
-            if np.linalg.norm(p - comp_first) > confine:
+            # |p - first| > confine, decided in plain floats away from the bound (np.linalg.norm at it)
+            dx, dy, dz = p[0] - comp_first[0], p[1] - comp_first[1], p[2] - comp_first[2]
+            r2 = dx * dx + dy * dy + dz * dz
+            if r2 > conf2_hi or (r2 >= conf2_lo and np.linalg.norm(p - comp_first) > confine):
                 return False
         cand = [j for j in grid.near(p) if not (chain_start <= j and gidx - 3 <= j < gidx)]
         if not cand:
@@ -143,6 +193,7 @@ def _walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size):
                 if placed % compound_size != 0:
                     comp_first = pos[start + (placed // compound_size) * compound_size]
                 success = False
+                frame = _frame(pos[g - 3], pos[g - 2], pos[g - 1]) if placed >= 3 else None
                 for _ in range(40):
                     if placed == 0:
                         p = rng.uniform(0, L, 3)
@@ -152,8 +203,7 @@ def _walk_chains(rng, n_chains, n_beads_chain, L, confine, compound_size):
                         a = pos[g - 1] + _random_unit(rng)
                         p = _place(a, pos[g - 2], pos[g - 1], BOND_B0, ANGLE_THETA0, rng.uniform(0, 2 * math.pi))
                     else:
-                        p = _place(pos[g - 3], pos[g - 2], pos[g - 1], BOND_B0, ANGLE_THETA0,
-                                   rng.uniform(0, 2 * math.pi))
+                        p = _place_in(pos[g - 1], frame, BOND_B0, ANGLE_THETA0, rng.uniform(0, 2 * math.pi))
                     if ok(p, g, start, comp_first):
                         success = True
                         break
@@ -195,7 +245,8 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
                  n_solvents: int | None, T0: float = 300.0, dt: float = DT,
                  solvent_density: float = 33.0, compound_size: int = COMPOUND_SIZE,
                  forcefield: np.ndarray | None = None, confine: float | None = None,
-                 central_first: bool = False, sphere_margin: float = 0.0) -> LimaSystem:
+                 central_first: bool = False, sphere_margin: float = 0.0,
+                 auto_density: bool = False) -> LimaSystem:
     """Build one seeded system.  n_solvents=None keeps every free lattice site.
 
     confine: largest distance of a compound's particles from its first walk bead
@@ -203,7 +254,9 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
     each compound from the bead nearest its centroid, so that the particle every
     other one is re-imaged against (Engine.cu:993-996) sits in the middle.
     sphere_margin: added to each compound's confining sphere (Bodies.cuh:407-423),
-    which widens its neighbour-list radii for chains that swell during the run."""
+    which widens its neighbour-list radii for chains that swell during the run.
+    auto_density: when fewer lattice sites than n_solvents are free of the chain, rebuild with a
+    lattice 10% denser (same seed), until enough are."""
     rng = np.random.default_rng(seed)
     ff = make_forcefield(rng, n_types) if forcefield is None else forcefield.astype(np.float32)
     L = float(box_len)
@@ -320,21 +373,26 @@ def build_system(seed: int, n_chains: int, compounds_per_chain: int, box_len: fl
     ig_l = np.array([e[1] for lst in ignores for e in lst], dtype=np.int32)
 
     # ---- solvent lattice ----
-    nl = max(1, int(round(L * solvent_density ** (1.0 / 3.0))))
-    a = L / nl
-    g = (np.arange(nl) + 0.5) * a
-    sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
-    sites = sites + rng.uniform(-0.02, 0.02, size=sites.shape)
-    sites = np.mod(sites, L)
-    if n_chain > 0:
-        from scipy.spatial import cKDTree
-        wrapped = np.mod(cpos.astype(np.float64), L)
-        wrapped[wrapped >= L] = 0.0
-        tree = cKDTree(wrapped, boxsize=L)
-        sites_c = sites.copy()
-        sites_c[sites_c >= L] = 0.0
-        d, _ = tree.query(sites_c, k=1)
-        sites = sites[d >= SOLVENT_EXCLUSION]
+    density = solvent_density
+    while True:
+        nl = max(1, int(round(L * density ** (1.0 / 3.0))))
+        a = L / nl
+        g = (np.arange(nl) + 0.5) * a
+        sites = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+        sites = sites + rng.uniform(-0.02, 0.02, size=sites.shape)
+        sites = np.mod(sites, L)
+        if n_chain > 0:
+            from scipy.spatial import cKDTree
+            wrapped = np.mod(cpos.astype(np.float64), L)
+            wrapped[wrapped >= L] = 0.0
+            tree = cKDTree(wrapped, boxsize=L)
+            sites_c = sites.copy()
+            sites_c[sites_c >= L] = 0.0
+            d, _ = tree.query(sites_c, k=1)
+            sites = sites[d >= SOLVENT_EXCLUSION]
+        if n_solvents is None or n_solvents <= len(sites) or not auto_density:
+            break
+        density *= 1.1          # a finer lattice, until enough sites are free of the chain
     if n_solvents is not None:
         if n_solvents > len(sites):
             raise ValueError(f"only {len(sites)} free lattice sites for {n_solvents} solvents")
@@ -378,44 +436,72 @@ def small(seed_offset: int = 0) -> LimaSystem:
     return build_system(SEED_BASE + seed_offset, 1, 2, 5.0, 8, 3000, T0=300.0)
 
 
+# Compounds of the production and large systems are built within 1.6 nm of their first walk
+# bead, numbered from their central bead and given a 1.25 nm sphere margin: with these the
+# systems run their configured 10,000 and 5,000 steps (the compounds swell; the reference
+# re-images them against particle 0 and lists neighbours with spheres fixed at build time).
+LONG_RUN = dict(confine=1.6, central_first=True, sphere_margin=1.25)
+
+
 def production() -> LimaSystem:
     """10 chains x 10 compounds x 128 (12,800 chain) + lattice solvent at 33/nm^3, 10 nm, 16 types."""
-    return build_system(SEED_BASE + 1000, 10, 10, 10.0, 16, None, T0=300.0)
+    return build_system(SEED_BASE + 1000, 10, 10, 10.0, 16, None, T0=300.0, **LONG_RUN)
 
 
 def large() -> LimaSystem:
     """10 chains x 40 compounds x 128 (51,200 chain) + lattice solvent at 33/nm^3, 18 nm, 16 types."""
-    return build_system(SEED_BASE + 2000, 10, 40, 18.0, 16, None, T0=300.0)
+    return build_system(SEED_BASE + 2000, 10, 40, 18.0, 16, None, T0=300.0, **LONG_RUN)
 
 
-REPLICA_COMPOUND = 64      # MAX_COMPOUND_PARTICLES (Constants.cuh:58)
+# Compound size by box edge: 16 particles in boxes under 4.6 nm, else 32.  A compound of n
+# particles straightened to all-trans torsions reaches 0.122 n / 2 nm from its central bead
+# (1.0 nm at 16, 2.0 nm at 32), inside half of the smallest box it is given.
+REPLICA_COMPOUND = (16, 4.6, 32)
+REPLICA_CONFINE = 0.8      # nm, compound particles from their first walk bead
+REPLICA_MARGIN = 1.0       # nm, added to each compound's confining sphere
+REPLICA_BOX = (3.0, 6.0)   # nm, box edge range
 
 
-def replicas(n: int = 256, seed_config: int = 3) -> list[LimaSystem]:
-    """Uneven replicas: chain 128-1024 (2-16 compounds of 64), solvent 500-6000, each with its own
-    thermostat target U(280, 320) K.
+def _replica(args) -> LimaSystem:
+    r, seed_config, compound, confine, margin = args
+    seed = SEED_BASE + 1000 * seed_config + r
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    n_chain = COMPOUND_SIZE * int(rng.integers(1, 9))      # 128 .. 1024
+    nsol = int(rng.integers(500, 6001))
+    T0 = float(rng.uniform(280.0, 320.0))
+    L = ((n_chain + nsol) / 33.0) ** (1.0 / 3.0)
+    L = float(min(REPLICA_BOX[1], max(REPLICA_BOX[0], L)))
+    if isinstance(compound, tuple):      # (size in boxes under edge, edge, size otherwise)
+        compound = compound[0] if L < compound[1] else compound[2]
+    return build_system(seed, 1, n_chain // compound, L, 16, nsol, T0=T0, compound_size=compound,
+                        confine=confine, central_first=True, sphere_margin=margin, auto_density=True)
 
-    The chains of this force field stretch (the 1-5 LJ repulsion at 0.15 nm bonds favours trans
-    torsions), and the reference re-images every compound against its particle 0 and lists
-    neighbours with spheres fixed at build time.  So that a replica stays valid for ~3,000 steps,
-    compounds hold 64 particles, are numbered from their central bead and confined to 1.3 nm of
-    their first bead, their spheres carry a 1.5 nm margin, and the box edge scaled to density
-    33/nm^3 is kept within [5.5, 6] nm."""
-    out = []
-    for r in range(n):
-        seed = SEED_BASE + 1000 * seed_config + r
-        rng = np.random.default_rng(seed ^ 0x5EED)
-        ncomp = int(rng.integers(1, 9))                 # chain of 128 * ncomp particles
-        nsol = int(rng.integers(500, 6001))
-        T0 = float(rng.uniform(280.0, 320.0))
-        nc = ncomp * COMPOUND_SIZE // REPLICA_COMPOUND
-        L = ((nc * REPLICA_COMPOUND + nsol) / 33.0) ** (1.0 / 3.0)
-        L = float(min(6.0, max(5.5, L)))
-        cap = int(0.8 * 33.0 * L ** 3) - nc * REPLICA_COMPOUND * 3
-        nsol = max(100, min(nsol, cap))
-        out.append(build_system(seed, 1, nc, L, 16, nsol, T0=T0, compound_size=REPLICA_COMPOUND, confine=1.3,
-                                central_first=True, sphere_margin=1.5))
-    return out
+
+def replicas(n: int = 256, seed_config: int = 3, workers: int | None = None, compound=REPLICA_COMPOUND,
+             confine: float = REPLICA_CONFINE, margin: float = REPLICA_MARGIN) -> list[LimaSystem]:
+    """Uneven replicas: chain 128-1024 particles, solvent 500-6000, box edge scaled to 33 nm^-3 and
+    held within 3-6 nm, each replica with its own seed and thermostat target U(280, 320) K.
+
+    The chains of this force field straighten (only 1-2, 1-3 and 1-4 pairs are excluded, and the
+    1-5 LJ repulsion at 0.15 nm bonds drives every torsion to trans over the weak dihedral), and
+    the reference re-images every compound against its particle 0 and lists neighbours with
+    spheres fixed at build time.  So that every replica stays valid for the run (4,000 steps
+    measured, twice the configured 2,000), compounds hold 16 particles in boxes under 4.6 nm and
+    32 otherwise (`compound`: an int, or (size under edge, edge, size otherwise)), are numbered
+    from their central bead, are built within 0.8 nm of their first bead, and their spheres
+    carry a 1.0 nm margin.  Where fewer lattice sites than the drawn solvent count are free of
+    the chain, the solvent lattice is made finer.  Built in `workers` processes (default: up to
+    16)."""
+    import os
+    args = [(r, seed_config, compound, confine, margin) for r in range(n)]
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and n >= 8:
+        import multiprocessing
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(workers, mp_context=multiprocessing.get_context("spawn")) as ex:
+            return list(ex.map(_replica, args, chunksize=2))
+    return [_replica(a) for a in args]
 
 
 def adjoint_replicas(n: int = 64) -> list[LimaSystem]:
