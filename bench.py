@@ -6,7 +6,8 @@ particles (12,800 chain particles) plus lattice solvent at 33 nm^-3 (~24k) in a
 Berendsen thermostat, FP32 pairs.  Synthetic seeded data, generated here.
 
 Timing: W warmup steps (the first captures the launch graphs), then untimed steps up to the next
-rebuild-cycle boundary (reported as warmup_align_steps), then exactly K timed steps.
+rebuild-cycle boundary (warmup_align_steps) and whole cycles for 50 ms while the GPU clocks rise
+(warmup_ramp_steps), then exactly K timed steps.
 
 Prints ONE JSON line: ns/day (whole job), ms/step, the roofline of the dominant
 kernel (the all-pairs solvent<-chain LJ kernel, VALU FP32 bound), and the CPU
@@ -24,6 +25,7 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PAIR = 23          # algorithmic FP32 flops per directed LJ pair (force only)
 PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 vector/matrix peak (packed); unpacked FMA rate is half
 PEAK_HBM_GBS = 8000.0
+RAMP_S = 0.05                # untimed stepping before the timed region, after warmup (clock ramp)
 
 
 def log(*a):
@@ -148,18 +150,22 @@ def cpu_baseline(systems_, params, budget_s):
 
 
 def sequential_baseline(syss, params, steps):
-    """The same replicas stepped one engine at a time (plain single-system launches)."""
+    """The same replicas stepped one after another, each in its own single-system engine (plain
+    launches of one system): one engine exists at a time; its warmup and creation are untimed,
+    and the timed steps of all replicas are summed."""
     from quantom_amd.engine import Engine
-    engines = [Engine([s], params) for s in syss]
-    for e in engines:
+    total = 0.0
+    for r, s in enumerate(syss):
+        e = Engine([s], params)
         e.step(params.nlist_interval)
-    t0 = time.perf_counter()
-    for e in engines:
+        e.sync()
+        t0 = time.perf_counter()
         e.step(steps)
-    t = time.perf_counter() - t0
-    for e in engines:
+        total += time.perf_counter() - t0
         e.close()
-    return 1000.0 * t / steps
+        if (r + 1) % 64 == 0:
+            log(f"sequential baseline: {r + 1} of {len(syss)} replicas")
+    return 1000.0 * total / steps
 
 
 def accuracy_report(args):
@@ -448,12 +454,20 @@ def main():
     if args.warmup > 1:
         eng.step(args.warmup - 1)
     # untimed steps up to the next rebuild-cycle boundary (state parity 0, a fresh list period),
-    # so that the timed steps replay whole captured cycles as a long run does; they are reported
+    # so that the timed steps replay whole captured cycles as a long run does, then whole cycles
+    # until the GPU has stepped for RAMP_S: its clocks rise over the first ~25 ms of continuous
+    # work (20-step windows measured 0.280 -> 0.244 ms/step on production), so a short timed
+    # region right after a short warmup would time the ramp.  Both counts are reported.
     iv = max(1, params.nlist_interval)
     cycle = iv if iv % 2 == 0 else 2 * iv
     align = (-eng.steps) % cycle
     if align:
         eng.step(align)
+    ramp = 0
+    t_r = time.perf_counter()
+    while time.perf_counter() - t_r < RAMP_S:
+        eng.step(cycle)
+        ramp += cycle
     eng.sync()
     clk0 = eng.kernel_clock()
     t0 = time.perf_counter()
@@ -522,6 +536,9 @@ def main():
         "kernels_us": kt,
         "first_call_ms": first_call_ms,
         "warmup_align_steps": align,
+        "warmup_ramp_steps": ramp,
+        "warmup_note": (f"untimed: the {args.warmup} warmup steps, {align} steps to a rebuild-cycle boundary, "
+                        f"then {ramp} steps (whole cycles, {RAMP_S * 1000:.0f} ms) while the GPU clocks rise"),
         "drop_in_calls": {"steps": n_pc, "ms_per_step_queued": per_call_ms, "ms_per_step_waiting": per_call_sync_ms,
                           "pattern": "one lima_step_async(1) per step, lima_sync every list period "
                                      "(lima::Engine::deviceMaster/hostMaster); beside it lima_step(1) per step"},
@@ -531,6 +548,7 @@ def main():
         out["value"] = ns_day * len(syss)          # whole job: simulated ns of all replicas per day
         out["unit"] = "replica-ns/day"
         if args.sequential:
+            eng.close()
             ms_seq = sequential_baseline(syss, params, max(10, args.steps // 10))
             out["sequential_ms_per_step"] = ms_seq
             out["batched_over_sequential"] = ms_seq / ms

@@ -545,6 +545,8 @@ struct lima_engine {
         hipGraphExec_t ex = nullptr;
         HIP_TRY(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
         HIP_TRY(hipGraphDestroy(g));
+        // the one-time upload of the executable graph here, not inside its first replay
+        HIP_TRY(hipGraphUpload(ex, stream));
         return ex;
     }
 
