@@ -134,6 +134,7 @@ struct lima_engine {
     int R = 0;
     std::vector<ReplicaInfo> reps_h;
     int NC = 0, NCH = 0, NS = 0, NB = 0;
+    int max_cn = 0, max_cb = 0, max_ca = 0, max_cd = 0;   // most particles, bonds, angles, dihedrals of a compound
     std::vector<int> sys_chain_begin, sys_solv_begin, sys_comp_begin;  // per system, into the concatenation
     int n_cells = 0, n_stiles = 0;
     int kmax = 0;                 // max solvent<-chain chunks in any replica
@@ -173,6 +174,8 @@ struct lima_engine {
     DBuf<int> cc_cnt, cc_ids, cs_cnt, cs_ids, ss_cnt, ss_ids, seg_cnt;
     int max_segs = 1;
     DBuf<int> cw_off, cw_total;
+    DBuf<int> item_comp;          // [max_slots] compound of each work item
+    DBuf<float2> tab_inter_ab;    // [R*1024] lj_ab of tab_inter
     DBuf<float4> partC, partS, cloc;
     DBuf<I3> partC_fx, partS_fx, cloc_fx;
     DBuf<SCWork> scwork;
@@ -287,6 +290,7 @@ struct lima_engine {
         a.stiles = stiles.p;
         a.mass = mass.p;
         a.tab_sc = tab_sc.p; a.tab_cs = tab_cs.p; a.tab_inter = tab_inter.p; a.tab_intra = tab_intra.p; a.tab_ss = tab_ss.p;
+        a.tab_inter_ab = tab_inter_ab.p;
         a.cap_cc = params.max_compound_neighbors;
         a.cap_cs = params.max_compound_solvents;
         a.cap_ss = params.max_solvent_neighbors;
@@ -294,7 +298,7 @@ struct lima_engine {
         a.cs_cnt = cs_cnt.p; a.cs_ids = cs_ids.p;
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
-        a.cw_off = cw_off.p; a.cw_total = cw_total.p;
+        a.cw_off = cw_off.p; a.cw_total = cw_total.p; a.item_comp = item_comp.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.cloc = cloc.p; a.cloc_fx = cloc_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
@@ -315,6 +319,7 @@ struct lima_engine {
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
         a.enable_dihedrals = params.enable_dihedrals;
+        a.loc_nb = max_cb; a.loc_na = max_ca; a.loc_nd = max_cd;
         return a;
     }
 
@@ -340,7 +345,7 @@ struct lima_engine {
         a.cell_ids = cell_ids.p; a.solv_cell = solv_cell.p;
         a.cell_rep = cell_rep.p;
         a.comp_cs_bound = comp_cs_bound.p;
-        a.cw_off = cw_off.p; a.cw_total = cw_total.p;
+        a.cw_off = cw_off.p; a.cw_total = cw_total.p; a.item_comp = item_comp.p;
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
         return a;
@@ -409,6 +414,23 @@ struct lima_engine {
         scwork.upload(scwork_h);
     }
 
+    // compound_local_kernel: one wavefront per compound when every compound fits one, else two;
+    // term results in dynamic LDS sized to the largest compound
+    template <bool FIXED, bool ENERGY> void launch_compound_local(const StepArgs& a, hipStream_t st) {
+        if (max_cn <= 64)
+            compound_local_kernel<FIXED, ENERGY, 64><<<NC, 64, local_lds_bytes<64>(max_cb, max_ca, max_cd), st>>>(a);
+        else
+            compound_local_kernel<FIXED, ENERGY, 128><<<NC, 128, local_lds_bytes<128>(max_cb, max_ca, max_cd), st>>>(a);
+    }
+
+    // compound_listed_kernel, or its packed form when every compound fits a group of 16, 32 or 64 lanes
+    template <bool FIXED, bool ENERGY> void launch_compound_listed(const StepArgs& a, int grid, hipStream_t st) {
+        if (max_cn <= 16) compound_listed_packed_kernel<FIXED, ENERGY, 16><<<grid, kCompoundBlock, 0, st>>>(a);
+        else if (max_cn <= 32) compound_listed_packed_kernel<FIXED, ENERGY, 32><<<grid, kCompoundBlock, 0, st>>>(a);
+        else if (max_cn <= 64) compound_listed_packed_kernel<FIXED, ENERGY, 64><<<grid, kCompoundBlock, 0, st>>>(a);
+        else compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, st>>>(a);
+    }
+
     // one force pass (and integration when ADVANCE) as a fork/join over four streams
     template <bool ADVANCE, bool FIXED, bool ENERGY> void launch_forces(int parity) {
         StepArgs a = step_args(parity);
@@ -423,7 +445,7 @@ struct lima_engine {
         HIP_TRY(hipStreamWaitEvent(s1, ev_fork, 0));
         if (NC > 0) {
             const int grid = cfg.listed_grid > 0 ? cfg.listed_grid : 4096;
-            compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, s1>>>(a);
+            launch_compound_listed<FIXED, ENERGY>(a, grid, s1);
         }
         // s2: the listed solvent-solvent LJ
         HIP_TRY(hipStreamWaitEvent(s2, ev_fork, 0));
@@ -437,7 +459,7 @@ struct lima_engine {
         // s3: bridge carry-in forces, then the compounds' local forces (no list needed)
         HIP_TRY(hipStreamWaitEvent(s3, ev_fork, 0));
         if (NB > 0) bridge_kernel<ENERGY><<<NB, 64, 0, s3>>>(a);
-        if (NC > 0) compound_local_kernel<FIXED, ENERGY><<<NC, kCompoundBlock, 0, s3>>>(a);
+        if (NC > 0) launch_compound_local<FIXED, ENERGY>(a, s3);
         HIP_TRY(hipEventRecord(ev_bridge, s3));
         HIP_TRY(hipStreamWaitEvent(s1, ev_bridge, 0));
         if (NC > 0) compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, s1>>>(a);
@@ -490,7 +512,21 @@ struct lima_engine {
             nl_solvent_solvent_kernel<<<xcd_grid(n_cells), kSolventListThreads, 0, stream>>>(a);
         }
         HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
-        if (NC > 0) nl_work_kernel<<<1, 1024, 0, stream>>>(a);
+        launch_work_list(a, stream);
+    }
+
+    // the listed work items of the current lists: per-compound offsets, then each item's compound
+    void launch_work_list(const NlistArgs& a, hipStream_t st) {
+        if (NC <= 0) return;
+        nl_work_kernel<<<1, 1024, 0, st>>>(a);
+        nl_item_kernel<<<NC, 64, 0, st>>>(a);
+    }
+
+    // lj_ab of the inter-compound tables (after any change of them)
+    void fill_ab_tables() {
+        const int n = R * kMaxTypes * kMaxTypes;
+        ab_table_kernel<<<(n + 255) / 256, 256, 0, stream>>>(tab_inter.p, tab_inter_ab.p, n);
+        HIP_TRY(hipStreamSynchronize(stream));
     }
 
     int cycle_steps() const {
@@ -759,6 +795,10 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
             const int ndh = s.dihedral_offset[c + 1] - s.dihedral_offset[c];
             if (nbd > kMaxBondsPerCompound || nan > kMaxAnglesPerCompound || ndh > kMaxDihedralsPerCompound)
                 throw LimaError(LIMA_E_CAPACITY, "too many bonded terms in a compound");
+            E->max_cn = std::max(E->max_cn, n);
+            E->max_cb = std::max(E->max_cb, nbd);
+            E->max_ca = std::max(E->max_ca, nan);
+            E->max_cd = std::max(E->max_cd, ndh);
             std::vector<std::vector<uint32_t>> pinc(n);
             for (int t = 0; t < nbd; t++) {
                 const lima_pair_bond& b = s.bonds[s.bond_offset[c] + t];
@@ -920,6 +960,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->stiles.upload(stl);
     E->mass.upload(mass);
     E->tab_sc.upload(tsc); E->tab_cs.upload(tcs); E->tab_inter.upload(tinter); E->tab_intra.upload(tintra); E->tab_ss.upload(tss);
+    E->tab_inter_ab.alloc(tinter.size());
     const size_t NCx = std::max(E->NC, 1), NSx = std::max(E->NS, 1), NCHx = std::max(E->NCH, 1);
     E->cc_cnt.alloc(NCx); E->cc_cnt.zero();
     E->cc_ids.alloc(NCx * P.max_compound_neighbors);
@@ -936,6 +977,7 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->cw_off.alloc(NCx + 1); E->cw_off.zero();
     E->cw_total.alloc(1); E->cw_total.zero();
     E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
+    E->item_comp.alloc(std::max<size_t>(E->max_slots, 1)); E->item_comp.zero();
     if (P.accumulation == LIMA_ACCUM_FIXED64) {
         E->partC_fx.alloc(E->max_slots * kCompoundBlock);
         E->partC.alloc(1);
@@ -1008,7 +1050,7 @@ void load_lists(lima_engine* E, const int* src) {
     d2d(E->cs_ids.p, src, nc * P.max_compound_solvents * 4, st); src += nc * P.max_compound_solvents;
     d2d(E->ss_cnt.p, src, ns * 4, st); src += ns;
     d2d(E->ss_ids.p, src, ns * E->ss_ld() * 4, st);
-    if (E->NC > 0) nl_work_kernel<<<1, 1024, 0, st>>>(E->nlist_args(E->cur));   // listed work list of these lists
+    E->launch_work_list(E->nlist_args(E->cur), st);   // listed work list of these lists
 }
 size_t cc_words(const lima_engine* E) { return (size_t)E->NC * (1 + E->params.max_compound_neighbors); }
 void save_cc(lima_engine* E, int* dst) {
@@ -1547,6 +1589,7 @@ void set_parameters(lima_engine* E, int r, const double* v) {
     HIP_TRY(hipMemcpy(E->tab_sc.p + r * kMaxTypes, tsc.data(), kMaxTypes * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(E->tab_cs.p + r * kMaxTypes, tcs.data(), kMaxTypes * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(E->tab_inter.p + (size_t)r * 1024, tinter.data(), 1024 * 16, hipMemcpyHostToDevice));
+    E->fill_ab_tables();
     HIP_TRY(hipMemcpy(E->tab_intra.p + (size_t)r * 1024, tintra.data(), 1024 * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(E->tab_ss.p + r, tss.data(), 16, hipMemcpyHostToDevice));
     for (int kind = 0; kind < 2; kind++) {
@@ -1714,6 +1757,7 @@ int lima_create(const lima_system* systems, int32_t n_systems, const lima_run_pa
         E->scratch_d = scratch_address(E->device);
         E->create_streams();
         validate_and_build(E, systems, n_systems);
+        E->fill_ab_tables();
         // Engine::Engine -> handleNLISTS(force) (Engine.cu:35)
         E->launch_rebuild(E->cur);
         HIP_TRY(hipStreamSynchronize(E->stream));
@@ -2083,8 +2127,7 @@ int lima_set_neighbors(lima_engine* E, const int32_t* ccc, const int32_t* cci, c
             HIP_TRY(hipMemcpy(E->ss_ids.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
         }
         if (E->NC) {
-            NlistArgs a = E->nlist_args(E->cur);
-            nl_work_kernel<<<1, 1024, 0, E->stream>>>(a);
+            E->launch_work_list(E->nlist_args(E->cur), E->stream);
             HIP_TRY(hipStreamSynchronize(E->stream));
         }
         return LIMA_OK;
@@ -2143,16 +2186,16 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
         if (E->NC > 0) {
             const int grid = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
             out6[1] = timeit([&] {
-                if (fx) compound_listed_kernel<true, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
-                else compound_listed_kernel<false, false><<<grid, kCompoundBlock, 0, E->stream>>>(a);
+                if (fx) E->launch_compound_listed<true, false>(a, grid, E->stream);
+                else E->launch_compound_listed<false, false>(a, grid, E->stream);
             });
             out6[2] = timeit([&] {
                 if (fx) compound_integrate_kernel<false, true, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
                 else compound_integrate_kernel<false, false, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
             });
             out8[7] = timeit([&] {
-                if (fx) compound_local_kernel<true, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
-                else compound_local_kernel<false, false><<<E->NC, kCompoundBlock, 0, E->stream>>>(a);
+                if (fx) E->launch_compound_local<true, false>(a, E->stream);
+                else E->launch_compound_local<false, false>(a, E->stream);
             });
         }
         if (E->params.enable_solvents && E->NS > 0) {

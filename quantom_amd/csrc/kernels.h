@@ -75,6 +75,8 @@ struct StepArgs {
     float4* ss_part;              // solvent-solvent local sums
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
+    const int* item_comp;         // [work item] its compound (written with cw_off at each rebuild)
+    const float2* tab_inter_ab;   // [R*1024] lj_ab of tab_inter, [source type * 32 + target type]
     // partial forces
     float4* partC;  I3* partC_fx;   // [slot*128 + p]
     float4* cloc;   I3* cloc_fx;    // [n_chain] bridge + bonded + intra-compound LJ (compound_local_kernel)
@@ -98,6 +100,7 @@ struct StepArgs {
     double* energy;
     uint32_t* err;                // [4]
     int enable_solvents, enable_dihedrals;
+    int loc_nb, loc_na, loc_nd;   // most bonds, angles, dihedrals of any compound (compound_local LDS strides)
 };
 
 enum { kE_bond = 0, kE_angle, kE_dih, kE_intra, kE_inter, kE_cs, kE_sc, kE_ss, kE_virial, kE_kin, kEnergyTerms };
@@ -372,23 +375,20 @@ template <bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
     __shared__ float4 src[kSolventChunk];
     __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (A, B) of lj_ab: 8-byte entries keep the per-lane reads conflict-light
-    __shared__ float4 xch[kCompoundBlock / 2];      // the second wave's partial sums of a split item
     __shared__ int cached_rep;
     const int p = threadIdx.x;
     if (p == 0) cached_rep = -1;
     const int total = *a.cw_total;
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
-        const int c = find_compound(a.cw_off, a.n_comp, w);
+        const int c = a.item_comp[w];
         const int k = w - a.cw_off[c];
         const int r = a.comp_rep[c];
         const ReplicaInfo rp = a.reps[r];
         const float L = rp.L, half = rp.half;
         const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
-        // a compound of at most one wavefront's particles: both waves take particle pi, each
-        // one contiguous half of the item's sources; the halves are added in order at the end
-        const bool split = n <= kCompoundBlock / 2;
-        const int pi = split ? (p & (kCompoundBlock / 2 - 1)) : p;
-        const int hs = split ? (p / (kCompoundBlock / 2)) : 0;
+        // one thread per particle sums the item's sources whole, in list order (as the packed
+        // form does, so that a replica gives the same sums in any batch)
+        const int pi = p;
         const bool active = pi < n;
         float4 me = a.cpos[base + (active ? pi : 0)];
         const float4 p0 = a.cpos[base];
@@ -433,8 +433,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             // exclusions across compounds exist only between bridged compounds: a wave
             // with none against this neighbour takes the loop without the mask test
             const bool wave_clean = __ballot((m0 | m1 | m2 | m3) != 0u) == 0ull;
-            const int hn = split ? (nn + 1) / 2 : nn;
-            const int jlo = hs * hn, jhi = min(nn, jlo + hn);
+            const int jlo = 0, jhi = nn;
             if (active && wave_clean) {
                 const float2* trow = tab + tp;
 #pragma unroll 4
@@ -492,8 +491,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             __syncthreads();
             const float4 pv = a.tab_cs[r * kMaxTypes + tp];
             const float2 pr = lj_ab(pv.x, pv.y);
-            const int hn = split ? (nsrc + 1) / 2 : nsrc;
-            const int jlo = hs * hn, jhi = min(nsrc, jlo + hn);
+            const int jlo = 0, jhi = nsrc;
             if (active) {
                 for (int j = jlo; j < jhi; j++) {
                     const float4 q = src[j];
@@ -506,21 +504,7 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                 block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
             }
         }
-        if (split) {
-            if (hs == 1) xch[pi] = make_float4(lx, ly, lz, 0.f);
-            __syncthreads();
-            if (hs == 0 && active) {
-                const float4 o = xch[pi];
-                const size_t slot = (size_t)w * kCompoundBlock + pi;
-                if (FIXED) {
-                    uint32_t bad = 0;
-                    a.partC_fx[slot] = I3{to_fixed(lx, bad) + to_fixed(o.x, bad), to_fixed(ly, bad) + to_fixed(o.y, bad),
-                                          to_fixed(lz, bad) + to_fixed(o.z, bad)};
-                    if (bad) atomicOr(a.err, bad);
-                } else
-                    a.partC[slot] = make_float4(lx + o.x, ly + o.y, lz + o.z, 0.f);
-            }
-        } else if (active) {
+        if (active) {
             const size_t slot = (size_t)w * kCompoundBlock + p;
             if (FIXED) {
                 uint32_t bad = 0;
@@ -532,6 +516,160 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// compound_listed_kernel for engines whose compounds all hold at most NPAD particles
+// (NPAD 16, 32 or 64): a block takes G = 128 / NPAD consecutive work items at once, one per
+// group of NPAD lanes, lane pi of group g being particle pi of item w0 + g's compound.  Each
+// group stages its item's sources in its own LDS row and every lane sums them whole in list
+// order, so a work item is still one thread per particle and writes the same partial slots as
+// compound_listed_kernel; small compounds no longer leave most lanes idle.  Groups of one
+// block may belong to different replicas, so the inter-compound pair constants come from the
+// precomputed global table tab_inter_ab rather than a per-block LDS copy.
+// ---------------------------------------------------------------------------
+template <int NPAD> LIMA_DEV double group_sum(double v) {
+#pragma unroll
+    for (int off = NPAD / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <bool FIXED, bool ENERGY, int NPAD>
+__global__ __launch_bounds__(128) void compound_listed_packed_kernel(StepArgs a) {
+    static_assert(NPAD == 16 || NPAD == 32 || NPAD == 64, "groups of 16, 32 or 64 lanes");
+    constexpr int G = kCompoundBlock / NPAD;
+    __shared__ float4 src[G][kSolventChunk];
+    const int p = threadIdx.x, g = p / NPAD, pi = p % NPAD;
+    const int total = *a.cw_total;
+    for (int w0 = blockIdx.x * G; w0 < total; w0 += gridDim.x * G) {
+        const int w = w0 + g;
+        const bool valid = w < total;
+        const int wi = valid ? w : w0;
+        const int c = a.item_comp[wi];
+        const int k = wi - a.cw_off[c];
+        const int r = a.comp_rep[c];
+        const ReplicaInfo rp = a.reps[r];
+        const float L = rp.L, half = rp.half;
+        const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
+        const bool active = valid && pi < n;
+        float4 me = a.cpos[base + (active ? pi : 0)];
+        const float4 p0 = a.cpos[base];
+        hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);
+        const int tp = ftype(me);
+        const bool is_cc = k < a.cc_cnt[c];
+        int nsrc = 0;
+        uint32_t m0 = 0, m1 = 0;
+        float2 pr_cs = make_float2(0.f, 0.f);
+        __syncthreads();   // every group is done with the previous round's sources
+        float4* sg = src[g];
+        if (valid) {
+            if (is_cc) {
+                // one neighbour compound, re-imaged against this compound's particle 0 (Engine.cu:949)
+                const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
+                const int nbase = a.comp_off[nb];
+                nsrc = a.comp_off[nb + 1] - nbase;
+                for (int j = pi; j < nsrc; j += NPAD) {
+                    float4 q = a.cpos[nbase + j];
+                    hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                    sg[j] = q;
+                }
+                // LJ_Ignores of this particle against the neighbour (Engine.cu:660)
+                if (active) {
+                    for (int e = a.xoff[base + pi]; e < a.xoff[base + pi + 1]; e++) {
+                        const int2 x = a.xent[e];
+                        if (x.x == nb) {
+                            const uint32_t bit = 1u << (x.y & 31);
+                            m0 |= x.y < 32 ? bit : 0u;
+                            m1 |= x.y >= 32 ? bit : 0u;
+                        }
+                    }
+                }
+            } else {
+                // 256 entries of the compound's solvent list (Engine.cu:967)
+                const int j0 = (k - a.cc_cnt[c]) * kSolventChunk;
+                nsrc = min(kSolventChunk, a.cs_cnt[c] - j0);
+                const int* ids = a.cs_ids + (size_t)c * a.cap_cs + j0;
+#pragma unroll
+                for (int u = 0; u < kSolventChunk / NPAD; u++) {
+                    const int jj = pi + u * NPAD;
+                    if (jj < nsrc) {
+                        float4 q = a.spos[ids[jj]];
+                        hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
+                        sg[jj] = q;
+                    }
+                }
+                const float4 pv = a.tab_cs[r * kMaxTypes + tp];
+                pr_cs = lj_ab(pv.x, pv.y);
+            }
+        }
+        __syncthreads();
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        double en = 0.0, vir = 0.0;
+        if (active) {
+            if (is_cc) {
+                const float2* tab = a.tab_inter_ab + (size_t)r * kMaxTypes * kMaxTypes + tp;
+                if ((m0 | m1) == 0u) {
+#pragma unroll 4
+                    for (int j = 0; j < nsrc; j++) {
+                        const float4 q = sg[j];
+                        const float2 pr = tab[ftype(q) * kMaxTypes];
+                        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                        const float r2 = pair_r2(dx, dy, dz);
+                        float inv;
+                        const float fs = lj_scalar(r2, pr.x, pr.y, inv);
+                        pair_add(dx, dy, dz, fs, lx, ly, lz);
+                        if (ENERGY) {
+                            en += lj_energy(inv, pr.x, pr.y);
+                            pair_virial(r2, fs, vir);
+                        }
+                    }
+                } else {
+                    for (int j = 0; j < nsrc; j++) {
+                        const float4 q = sg[j];
+                        const float2 pr = tab[ftype(q) * kMaxTypes];
+                        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                        const float r2 = pair_r2(dx, dy, dz);
+                        float inv;
+                        float fs = lj_scalar(r2, pr.x, pr.y, inv);
+                        const bool ex = (((j < 32 ? m0 : m1) >> (j & 31)) & 1u) != 0u;
+                        fs = ex ? 0.0f : fs;
+                        pair_add(dx, dy, dz, fs, lx, ly, lz);
+                        if (ENERGY && !ex) {
+                            en += lj_energy(inv, pr.x, pr.y);
+                            pair_virial(r2, fs, vir);
+                        }
+                    }
+                }
+            } else {
+#pragma unroll 4
+                for (int j = 0; j < nsrc; j++) {
+                    const float4 q = sg[j];
+                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr_cs.x, pr_cs.y, lx, ly, lz, en, vir);
+                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr_cs.x, pr_cs.y, lx, ly, lz);
+                }
+            }
+            const size_t slot = (size_t)w * kCompoundBlock + pi;
+            if (FIXED) {
+                uint32_t bad = 0;
+                a.partC_fx[slot] = I3{to_fixed(lx, bad), to_fixed(ly, bad), to_fixed(lz, bad)};
+                if (bad) atomicOr(a.err, bad);
+            } else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
+        }
+        if (ENERGY) {
+            en = group_sum<NPAD>(en);
+            vir = group_sum<NPAD>(vir);
+            if (valid && pi == 0) {
+                if (en != 0.0) atomicAdd(a.energy + r * kEnergyTerms + (is_cc ? kE_inter : kE_cs), en);
+                if (vir != 0.0) atomicAdd(a.energy + r * kEnergyTerms + kE_virial, vir);
+            }
+        }
+    }
+}
+
+// lj_ab of every inter-compound table entry: the global copy the packed listed kernel reads
+__global__ void ab_table_kernel(const float4* t, float2* ab, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ab[i] = lj_ab(t[i].x, t[i].y);
+}
+
+// ---------------------------------------------------------------------------
 // compoundKernel's local forces (Engine.cu:918-937, 678-700): bridge carry-in,
 // bonded terms (term forces computed once, then gathered per particle in term
 // index order = the serialized scatter's order, Engine.cu:765-771) and the
@@ -539,13 +677,25 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
 // needs no neighbour list, so it runs beside the listed kernel; the sum goes to
 // cloc (int64 in the fixed-point form).
 // ---------------------------------------------------------------------------
-template <bool FIXED, bool ENERGY>
-__global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
-    __shared__ float4 pos[kCompoundBlock];
-    __shared__ F3 rb[kMaxBondsPerCompound * 2];
-    __shared__ F3 ra[kMaxAnglesPerCompound * 3];
-    __shared__ F3 rd[kMaxDihedralsPerCompound * 4];
-    __shared__ float2 tab[kMaxTypes * kMaxTypes];
+// BLOCK threads (64 when every compound of the engine fits one wavefront, else 128); the term
+// results are staged in dynamic LDS sized to the engine's largest compound (local_lds_bytes):
+// [BLOCK] float4 positions, [1024] float2 pair constants, then the bond, angle and dihedral
+// results at strides a.loc_nb, a.loc_na, a.loc_nd terms.
+template <int BLOCK>
+inline size_t local_lds_bytes(int nb, int na, int nd) {
+    return (size_t)BLOCK * sizeof(float4) + (size_t)kMaxTypes * kMaxTypes * sizeof(float2) +
+           (size_t)(2 * nb + 3 * na + 4 * nd) * sizeof(F3);
+}
+
+template <bool FIXED, bool ENERGY, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void compound_local_kernel(StepArgs a) {
+    static_assert(BLOCK == 64 || BLOCK == 128, "one or two wavefronts");
+    extern __shared__ float4 dyn_lds[];
+    float4* pos = dyn_lds;
+    float2* tab = reinterpret_cast<float2*>(dyn_lds + BLOCK);
+    F3* rb = reinterpret_cast<F3*>(tab + kMaxTypes * kMaxTypes);
+    F3* ra = rb + 2 * a.loc_nb;
+    F3* rd = ra + 3 * a.loc_na;
     const int c = blockIdx.x;
     const int p = threadIdx.x;
     const int r = a.comp_rep[c];
@@ -560,12 +710,12 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
     pos[p] = me;
     {
         // a compile-time stride, so that the eight loads per lane are in flight together
-        float4 v[kMaxTypes * kMaxTypes / kCompoundBlock];
+        float4 v[kMaxTypes * kMaxTypes / BLOCK];
 #pragma unroll
-        for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++)
-            v[u] = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + p + u * kCompoundBlock];
+        for (int u = 0; u < kMaxTypes * kMaxTypes / BLOCK; u++)
+            v[u] = a.tab_intra[(size_t)r * kMaxTypes * kMaxTypes + p + u * BLOCK];
 #pragma unroll
-        for (int u = 0; u < kMaxTypes * kMaxTypes / kCompoundBlock; u++) tab[p + u * kCompoundBlock] = lj_ab(v[u].x, v[u].y);
+        for (int u = 0; u < kMaxTypes * kMaxTypes / BLOCK; u++) tab[p + u * BLOCK] = lj_ab(v[u].x, v[u].y);
     }
     __syncthreads();
     double eb = 0, ea = 0, ed = 0;
@@ -573,16 +723,16 @@ __global__ __launch_bounds__(128) void compound_local_kernel(StepArgs a) {
     const int a0 = a.angle_off[c], na = a.angle_off[c + 1] - a0;
     const int d0 = a.dih_off[c], nd = a.dih_off[c + 1] - d0;
     auto P = [&](int i) { float4 q = pos[i]; return f3(q.x, q.y, q.z); };
-    for (int t = p; t < nb; t += kCompoundBlock) {
+    for (int t = p; t < nb; t += BLOCK) {
         BondD bd = a.bonds[b0 + t];
         pairbond(P(bd.a0), P(bd.a1), bd.b0, bd.kb, &rb[t * 2], &eb);
     }
-    for (int t = p; t < na; t += kCompoundBlock) {
+    for (int t = p; t < na; t += BLOCK) {
         AngleD ad = a.angles[a0 + t];
         anglebond(P(ad.a0), P(ad.a1), P(ad.a2), ad.t0, ad.k, &ra[t * 3], &ea);
     }
     if (a.enable_dihedrals) {
-        for (int t = p; t < nd; t += kCompoundBlock) {
+        for (int t = p; t < nd; t += BLOCK) {
             DihD dd = a.dihs[d0 + t];
             dihedral(P(dd.a0), P(dd.a1), P(dd.a2), P(dd.a3), dd.p0, dd.k, dd.n, &rd[t * 4], &ed);
         }
@@ -1210,6 +1360,7 @@ struct NlistArgs {
     int* seg_cnt;                 // [n_comp * max_segs] compound-solvent members per solvent segment
     int max_segs;
     int* cw_off; int* cw_total;
+    int* item_comp;
     uint32_t* err;
     int enable_solvents;
 };
@@ -1594,6 +1745,12 @@ __global__ __launch_bounds__(1024) void nl_work_kernel(NlistArgs a) {
         run += slots(b + k);
     }
     if (threadIdx.x == 1023) { a.cw_off[n] = part[1023]; *a.cw_total = part[1023]; }
+}
+
+// the compound of every work item (one block per compound)
+__global__ __launch_bounds__(64) void nl_item_kernel(NlistArgs a) {
+    const int c = blockIdx.x;
+    for (int w = a.cw_off[c] + threadIdx.x; w < a.cw_off[c + 1]; w += blockDim.x) a.item_comp[w] = c;
 }
 
 }  // namespace lima
