@@ -336,3 +336,44 @@ def test_adjoint_replica_gradient_matches_fd_of_fp64_oracle():
     for i in idx:
         assert abs(g[i] - fd[i]) <= 2e-3 * abs(fd[i]) + 1e-4 * scale, f"parameter {i}: adjoint {g[i]} fd {fd[i]}"
     e.close()
+
+
+def test_adjoint_replica_200_frames_matches_converged_fd():
+    """The adjoint workload's full horizon: 200 frames with checkpoints every 20.  FP64 central
+    differences of the oracle's loss are converged there (steps of 1e-3 to 1e-6 relative agree
+    to ~4e-3); the adjoint must agree with the 1e-5 step to 1.5e-2 relative for two types' sigma
+    and epsilon and the four bonded constants."""
+    from quantom_amd import systems
+    from quantom_amd.engine import Engine
+    from tests.test_adjoint import PARAMS, base_params, oracle_frames, oracle_loss, used_types, with_params
+    K = 200
+    s = systems.adjoint_replicas(1)[0]
+    print(f"case: adjoint replica, {K} frames, checkpoints every 20", flush=True)
+    v0 = base_params(s)
+    vt = v0.copy()
+    rng = np.random.default_rng(0x51A7 + 4000)
+    types = used_types(s)
+    for t in types:
+        vt[t] *= 1.0 + 0.03 * rng.uniform(-1, 1)
+        vt[32 + t] *= 1.0 + 0.10 * rng.uniform(-1, 1)
+    fct, fst, et = oracle_frames(with_params(s, vt.astype(np.float32).astype(np.float64)), K)
+    bfc, bfs, bpot = oracle_frames(s, K)
+    beta = 1.0 / float((fct ** 2).sum() + (fst ** 2).sum())
+    alpha = beta * float(((bfc - fct) ** 2).sum() + ((bfs - fst) ** 2).sum()) / float(((bpot - et) ** 2).sum())
+    fct32, fst32 = fct.astype(np.float32), fst.astype(np.float32)
+    e = Engine([s], PARAMS)
+    loss, grad = e.adjoint(K, [alpha], [beta], et[:, None], fct32, fst32, checkpoint_interval=20)
+    e.close()
+    l0 = oracle_loss(s, K, alpha, beta, et, fct32, fst32)
+    print(f"loss: adjoint {loss[0]:.9e} oracle {l0:.9e}", flush=True)
+    assert abs(loss[0] - l0) <= 1e-3 * abs(l0)
+    for i in (types[1], types[5], 32 + types[1], 32 + types[5], 64, 65, 66, 67):
+        h = 1e-5 * max(abs(v0[i]), 1e-3)
+        vp, vm = v0.copy(), v0.copy()
+        vp[i] = np.float32(v0[i] + h)
+        vm[i] = np.float32(v0[i] - h)
+        fd = (oracle_loss(with_params(s, vp), K, alpha, beta, et, fct32, fst32) -
+              oracle_loss(with_params(s, vm), K, alpha, beta, et, fct32, fst32)) / (float(vp[i]) - float(vm[i]))
+        rel = abs(grad[0][i] - fd) / abs(fd)
+        print(f"param {i:2d}  adjoint {grad[0][i]: .6e}  fd {fd: .6e}  rel {rel:.2e}", flush=True)
+        assert rel <= 1.5e-2, f"parameter {i}: adjoint {grad[0][i]} fd {fd}"
