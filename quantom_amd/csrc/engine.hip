@@ -175,6 +175,7 @@ struct lima_engine {
     int max_segs = 1;
     DBuf<int> cw_off, cw_total;
     DBuf<int> item_comp;          // [max_slots] compound of each work item
+    DBuf<int> run_off, run_cs_off, run_total, run_comp;   // runs of the packed listed kernel (compounds of <= 64 particles)
     DBuf<float2> tab_inter_ab;    // [R*1024] lj_ab of tab_inter
     DBuf<float4> partC, partS, cloc;
     DBuf<I3> partC_fx, partS_fx, cloc_fx;
@@ -299,6 +300,7 @@ struct lima_engine {
         a.ss_cnt = ss_cnt.p; a.ss_ids = ss_ids.p; a.ss_ld = ss_ld();
         a.ss_part = ss_part.p; a.ss_img = ss_img.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p; a.item_comp = item_comp.p;
+        a.run_off = run_off.p; a.run_cs_off = run_cs_off.p; a.run_total = run_total.p; a.run_comp = run_comp.p;
         a.partC = partC.p; a.partC_fx = partC_fx.p;
         a.cloc = cloc.p; a.cloc_fx = cloc_fx.p;
         a.partS = partS.p; a.partS_fx = partS_fx.p;
@@ -346,6 +348,8 @@ struct lima_engine {
         a.cell_rep = cell_rep.p;
         a.comp_cs_bound = comp_cs_bound.p;
         a.cw_off = cw_off.p; a.cw_total = cw_total.p; a.item_comp = item_comp.p;
+        a.run_len = pack_ppl() ? cc_run(pack_ppl()) : 0;
+        a.run_off = run_off.p; a.run_cs_off = run_cs_off.p; a.run_total = run_total.p; a.run_comp = run_comp.p;
         a.err = err.p;
         a.enable_solvents = params.enable_solvents;
         return a;
@@ -423,12 +427,18 @@ struct lima_engine {
             compound_local_kernel<FIXED, ENERGY, 128><<<NC, 128, local_lds_bytes<128>(max_cb, max_ca, max_cd), st>>>(a);
     }
 
-    // compound_listed_kernel, or its packed form when every compound fits a group of 16, 32 or 64 lanes
+    // particles per lane of the packed listed kernel (compounds of at most 16, 32, 64 particles), 0: plain kernel
+    int pack_ppl() const { return max_cn <= 16 ? 1 : max_cn <= 32 ? 2 : max_cn <= 64 ? 4 : 0; }
+
+    // compound_listed_kernel, or its packed form when every compound fits 16 lanes x 1, 2 or 4 particles
     template <bool FIXED, bool ENERGY> void launch_compound_listed(const StepArgs& a, int grid, hipStream_t st) {
-        if (max_cn <= 16) compound_listed_packed_kernel<FIXED, ENERGY, 16><<<grid, kCompoundBlock, 0, st>>>(a);
-        else if (max_cn <= 32) compound_listed_packed_kernel<FIXED, ENERGY, 32><<<grid, kCompoundBlock, 0, st>>>(a);
-        else if (max_cn <= 64) compound_listed_packed_kernel<FIXED, ENERGY, 64><<<grid, kCompoundBlock, 0, st>>>(a);
-        else compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, st>>>(a);
+        // one-wavefront blocks: twice the blocks of the two-wavefront plain kernel's grid
+        switch (pack_ppl()) {
+            case 1: compound_listed_packed_kernel<FIXED, ENERGY, 1><<<2 * grid, 64, 0, st>>>(a); break;
+            case 2: compound_listed_packed_kernel<FIXED, ENERGY, 2><<<2 * grid, 64, 0, st>>>(a); break;
+            case 4: compound_listed_packed_kernel<FIXED, ENERGY, 4><<<2 * grid, 64, 0, st>>>(a); break;
+            default: compound_listed_kernel<FIXED, ENERGY><<<grid, kCompoundBlock, 0, st>>>(a);
+        }
     }
 
     // one force pass (and integration when ADVANCE) as a fork/join over four streams
@@ -978,6 +988,10 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->cw_total.alloc(1); E->cw_total.zero();
     E->max_slots = NCx * (size_t)(P.max_compound_neighbors + (P.max_compound_solvents + kSolventChunk - 1) / kSolventChunk);
     E->item_comp.alloc(std::max<size_t>(E->max_slots, 1)); E->item_comp.zero();
+    E->run_off.alloc(NCx + 1); E->run_off.zero();
+    E->run_cs_off.alloc(NCx + 1); E->run_cs_off.zero();
+    E->run_total.alloc(2); E->run_total.zero();
+    E->run_comp.alloc(std::max<size_t>(E->max_slots, 1)); E->run_comp.zero();
     if (P.accumulation == LIMA_ACCUM_FIXED64) {
         E->partC_fx.alloc(E->max_slots * kCompoundBlock);
         E->partC.alloc(1);

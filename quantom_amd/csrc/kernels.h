@@ -76,6 +76,7 @@ struct StepArgs {
     float4* ss_img;               // the solvent's position after the reference's re-imaging sequence
     const int* cw_off; const int* cw_total;
     const int* item_comp;         // [work item] its compound (written with cw_off at each rebuild)
+    const int* run_off; const int* run_cs_off; const int* run_total; const int* run_comp;   // packed listed kernel's runs
     const float2* tab_inter_ab;   // [R*1024] lj_ab of tab_inter, [source type * 32 + target type]
     // partial forces
     float4* partC;  I3* partC_fx;   // [slot*128 + p]
@@ -516,145 +517,223 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// compound_listed_kernel for engines whose compounds all hold at most NPAD particles
-// (NPAD 16, 32 or 64): a block takes G = 128 / NPAD consecutive work items at once, one per
-// group of NPAD lanes, lane pi of group g being particle pi of item w0 + g's compound.  Each
-// group stages its item's sources in its own LDS row and every lane sums them whole in list
-// order, so a work item is still one thread per particle and writes the same partial slots as
-// compound_listed_kernel; small compounds no longer leave most lanes idle.  Groups of one
-// block may belong to different replicas, so the inter-compound pair constants come from the
-// precomputed global table tab_inter_ab rather than a per-block LDS copy.
+// compound_listed_kernel for engines whose compounds all hold at most 16 * PPL particles
+// (PPL 1, 2 or 4 particles per lane).  A block is one wavefront of four groups of 16 lanes and
+// takes four runs at once, one per group.  A run is either one solvent-chunk work item (its
+// 256 sources staged and summed in two halves of 128), or up to cc_run(PPL) = 8 / PPL
+// consecutive neighbour-compound items of one compound (at most 128 sources, staged together).
+// Lane pi of a group holds particles pi, pi + 16, ... of the run's compound and sums each
+// item's sources whole in list order, so every work item is still one sequential sum per
+// particle written to its own partial slot, as in compound_listed_kernel.  A group reads only
+// its own LDS row, so wavefront-level ordering is the only synchronisation.  Groups may belong
+// to different replicas: the inter-compound pair constants come from the global table
+// tab_inter_ab.
 // ---------------------------------------------------------------------------
-template <int NPAD> LIMA_DEV double group_sum(double v) {
+constexpr int kPackLanes = 16;
+constexpr int kPackGroups = 64 / kPackLanes;
+constexpr int kPackRow = 128;                 // staged sources per group
+__host__ __device__ constexpr int cc_run(int ppl) { return kPackRow / (kPackLanes * ppl); }
+
+template <int W> LIMA_DEV double group_sum(double v) {
 #pragma unroll
-    for (int off = NPAD / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
-template <bool FIXED, bool ENERGY, int NPAD>
-__global__ __launch_bounds__(128) void compound_listed_packed_kernel(StepArgs a) {
-    static_assert(NPAD == 16 || NPAD == 32 || NPAD == 64, "groups of 16, 32 or 64 lanes");
-    constexpr int G = kCompoundBlock / NPAD;
-    __shared__ float4 src[G][kSolventChunk];
-    const int p = threadIdx.x, g = p / NPAD, pi = p % NPAD;
-    const int total = *a.cw_total;
-    for (int w0 = blockIdx.x * G; w0 < total; w0 += gridDim.x * G) {
-        const int w = w0 + g;
-        const bool valid = w < total;
-        const int wi = valid ? w : w0;
-        const int c = a.item_comp[wi];
-        const int k = wi - a.cw_off[c];
+LIMA_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool FIXED, bool ENERGY, int PPL>
+__global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) {
+    static_assert(PPL == 1 || PPL == 2 || PPL == 4, "16, 32 or 64 particles per compound");
+    constexpr int RUN = cc_run(PPL);
+    // rows padded by one entry: the four groups' rows start in different LDS banks
+    __shared__ float4 src[kPackGroups][kPackRow + 1];
+    __shared__ int run_nb[kPackGroups][RUN], run_off[kPackGroups][RUN + 1];
+    const int p = threadIdx.x, g = p / kPackLanes, pi = p % kPackLanes;
+    float4* sg = src[g];
+    const int n_ccr = a.run_total[0], total = n_ccr + a.run_total[1];
+    for (int s0 = blockIdx.x * kPackGroups; s0 < total; s0 += gridDim.x * kPackGroups) {
+        const int sr = s0 + g;
+        const bool valid = sr < total;
+        const int si = valid ? sr : s0;
+        const int c = a.run_comp[si];
+        const bool is_cc = si < n_ccr;
+        const int ncc = a.cc_cnt[c];
         const int r = a.comp_rep[c];
         const ReplicaInfo rp = a.reps[r];
         const float L = rp.L, half = rp.half;
         const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
-        const bool active = valid && pi < n;
-        float4 me = a.cpos[base + (active ? pi : 0)];
         const float4 p0 = a.cpos[base];
-        hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);
-        const int tp = ftype(me);
-        const bool is_cc = k < a.cc_cnt[c];
-        int nsrc = 0;
-        uint32_t m0 = 0, m1 = 0;
-        float2 pr_cs = make_float2(0.f, 0.f);
-        __syncthreads();   // every group is done with the previous round's sources
-        float4* sg = src[g];
-        if (valid) {
-            if (is_cc) {
-                // one neighbour compound, re-imaged against this compound's particle 0 (Engine.cu:949)
-                const int nb = a.cc_ids[(size_t)c * a.cap_cc + k];
-                const int nbase = a.comp_off[nb];
-                nsrc = a.comp_off[nb + 1] - nbase;
-                for (int j = pi; j < nsrc; j += NPAD) {
-                    float4 q = a.cpos[nbase + j];
-                    hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
-                    sg[j] = q;
-                }
-                // LJ_Ignores of this particle against the neighbour (Engine.cu:660)
-                if (active) {
-                    for (int e = a.xoff[base + pi]; e < a.xoff[base + pi + 1]; e++) {
-                        const int2 x = a.xent[e];
-                        if (x.x == nb) {
-                            const uint32_t bit = 1u << (x.y & 31);
-                            m0 |= x.y < 32 ? bit : 0u;
-                            m1 |= x.y >= 32 ? bit : 0u;
-                        }
-                    }
-                }
-            } else {
-                // 256 entries of the compound's solvent list (Engine.cu:967)
-                const int j0 = (k - a.cc_cnt[c]) * kSolventChunk;
-                nsrc = min(kSolventChunk, a.cs_cnt[c] - j0);
-                const int* ids = a.cs_ids + (size_t)c * a.cap_cs + j0;
+        float mx[PPL], my[PPL], mz[PPL];
+        int tp[PPL], xb[PPL], xe[PPL];
+        bool act[PPL];
 #pragma unroll
-                for (int u = 0; u < kSolventChunk / NPAD; u++) {
-                    const int jj = pi + u * NPAD;
-                    if (jj < nsrc) {
-                        float4 q = a.spos[ids[jj]];
-                        hyperpos(p0.x, p0.y, p0.z, q.x, q.y, q.z, L, half);
-                        sg[jj] = q;
-                    }
-                }
-                const float4 pv = a.tab_cs[r * kMaxTypes + tp];
-                pr_cs = lj_ab(pv.x, pv.y);
-            }
+        for (int q = 0; q < PPL; q++) {
+            const int pidx = pi + kPackLanes * q;
+            act[q] = valid && pidx < n;
+            const int gi = base + (act[q] ? pidx : 0);
+            float4 me = a.cpos[gi];
+            hyperpos(p0.x, p0.y, p0.z, me.x, me.y, me.z, L, half);                 // Engine.cu:923
+            mx[q] = me.x; my[q] = me.y; mz[q] = me.z;
+            tp[q] = ftype(me);
+            xb[q] = act[q] ? a.xoff[gi] : 0;
+            xe[q] = act[q] ? a.xoff[gi + 1] : 0;
         }
-        __syncthreads();
-        float lx = 0.f, ly = 0.f, lz = 0.f;
+        // the run's items: neighbour items w_first .. w_first + nitems - 1, or one solvent chunk
+        const int k0 = is_cc ? (si - a.run_off[c]) * RUN : ncc + (si - n_ccr - a.run_cs_off[c]);
+        const int nitems = is_cc ? min(RUN, ncc - k0) : 1;
+        const int w_first = a.cw_off[c] + k0;
         double en = 0.0, vir = 0.0;
-        if (active) {
-            if (is_cc) {
-                const float2* tab = a.tab_inter_ab + (size_t)r * kMaxTypes * kMaxTypes + tp;
-                if ((m0 | m1) == 0u) {
-#pragma unroll 4
-                    for (int j = 0; j < nsrc; j++) {
-                        const float4 q = sg[j];
-                        const float2 pr = tab[ftype(q) * kMaxTypes];
-                        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                        const float r2 = pair_r2(dx, dy, dz);
-                        float inv;
-                        const float fs = lj_scalar(r2, pr.x, pr.y, inv);
-                        pair_add(dx, dy, dz, fs, lx, ly, lz);
-                        if (ENERGY) {
-                            en += lj_energy(inv, pr.x, pr.y);
-                            pair_virial(r2, fs, vir);
-                        }
-                    }
-                } else {
-                    for (int j = 0; j < nsrc; j++) {
-                        const float4 q = sg[j];
-                        const float2 pr = tab[ftype(q) * kMaxTypes];
-                        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-                        const float r2 = pair_r2(dx, dy, dz);
-                        float inv;
-                        float fs = lj_scalar(r2, pr.x, pr.y, inv);
-                        const bool ex = (((j < 32 ? m0 : m1) >> (j & 31)) & 1u) != 0u;
-                        fs = ex ? 0.0f : fs;
-                        pair_add(dx, dy, dz, fs, lx, ly, lz);
-                        if (ENERGY && !ex) {
-                            en += lj_energy(inv, pr.x, pr.y);
-                            pair_virial(r2, fs, vir);
-                        }
-                    }
-                }
-            } else {
-#pragma unroll 4
-                for (int j = 0; j < nsrc; j++) {
-                    const float4 q = sg[j];
-                    if (ENERGY) lj_acc_energy(me.x, me.y, me.z, q.x, q.y, q.z, pr_cs.x, pr_cs.y, lx, ly, lz, en, vir);
-                    else lj_acc(me.x, me.y, me.z, q.x, q.y, q.z, pr_cs.x, pr_cs.y, lx, ly, lz);
-                }
-            }
-            const size_t slot = (size_t)w * kCompoundBlock + pi;
+        auto store = [&](int w, int q, float lx, float ly, float lz) {
+            const size_t slot = (size_t)w * kCompoundBlock + pi + kPackLanes * q;
             if (FIXED) {
                 uint32_t bad = 0;
                 a.partC_fx[slot] = I3{to_fixed(lx, bad), to_fixed(ly, bad), to_fixed(lz, bad)};
                 if (bad) atomicOr(a.err, bad);
             } else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
+        };
+        wave_sync();   // every group of the wavefront is done with the previous round's rows
+        if (valid && is_cc) {
+            // the run's neighbour compounds, re-imaged against this compound's particle 0 (Engine.cu:949)
+            if (pi < nitems) {
+                const int nb = a.cc_ids[(size_t)c * a.cap_cc + k0 + pi];
+                run_nb[g][pi] = nb;
+                run_off[g][pi + 1] = a.comp_off[nb + 1] - a.comp_off[nb];
+            }
+            wave_sync();
+            if (pi == 0) {
+                run_off[g][0] = 0;
+                for (int t = 0; t < nitems; t++) run_off[g][t + 1] += run_off[g][t];
+            }
+            wave_sync();
+            for (int t = 0; t < nitems; t++) {
+                const int nbase = a.comp_off[run_nb[g][t]], o = run_off[g][t], nn = run_off[g][t + 1] - o;
+                for (int j = pi; j < nn; j += kPackLanes) {
+                    float4 qv = a.cpos[nbase + j];
+                    hyperpos(p0.x, p0.y, p0.z, qv.x, qv.y, qv.z, L, half);
+                    sg[o + j] = qv;
+                }
+            }
+            wave_sync();
+            const float2* tab = a.tab_inter_ab + (size_t)r * kMaxTypes * kMaxTypes;
+            for (int t = 0; t < nitems; t++) {
+                const int nb = run_nb[g][t], j0 = run_off[g][t], j1 = run_off[g][t + 1];
+                // LJ_Ignores of each particle against this neighbour (Engine.cu:660); only
+                // particles at a bridged junction have entries
+                uint32_t m0[PPL], m1[PPL];
+                bool clean = true;
+#pragma unroll
+                for (int q = 0; q < PPL; q++) {
+                    m0[q] = m1[q] = 0u;
+                    for (int e = xb[q]; e < xe[q]; e++) {
+                        const int2 x = a.xent[e];
+                        if (x.x == nb) {
+                            const uint32_t bit = 1u << (x.y & 31);
+                            m0[q] |= x.y < 32 ? bit : 0u;
+                            m1[q] |= x.y >= 32 ? bit : 0u;
+                        }
+                    }
+                    clean = clean && (m0[q] | m1[q]) == 0u;
+                }
+                float lx[PPL], ly[PPL], lz[PPL];
+#pragma unroll
+                for (int q = 0; q < PPL; q++) lx[q] = ly[q] = lz[q] = 0.f;
+                if (clean) {
+#pragma unroll 2
+                    for (int j = j0; j < j1; j++) {
+                        const float4 qv = sg[j];
+                        const int ts = ftype(qv) * kMaxTypes;
+#pragma unroll
+                        for (int q = 0; q < PPL; q++) {
+                            const float2 pr = tab[ts + tp[q]];
+                            const float dx = qv.x - mx[q], dy = qv.y - my[q], dz = qv.z - mz[q];
+                            const float r2 = pair_r2(dx, dy, dz);
+                            float inv;
+                            const float fs = lj_scalar(r2, pr.x, pr.y, inv);
+                            pair_add(dx, dy, dz, fs, lx[q], ly[q], lz[q]);
+                            if (ENERGY && act[q]) {
+                                en += lj_energy(inv, pr.x, pr.y);
+                                pair_virial(r2, fs, vir);
+                            }
+                        }
+                    }
+                } else {
+                    for (int j = j0; j < j1; j++) {
+                        const float4 qv = sg[j];
+                        const int ts = ftype(qv) * kMaxTypes, jl = j - j0;
+#pragma unroll
+                        for (int q = 0; q < PPL; q++) {
+                            const float2 pr = tab[ts + tp[q]];
+                            const float dx = qv.x - mx[q], dy = qv.y - my[q], dz = qv.z - mz[q];
+                            const float r2 = pair_r2(dx, dy, dz);
+                            float inv;
+                            float fs = lj_scalar(r2, pr.x, pr.y, inv);
+                            const bool ex = (((jl < 32 ? m0[q] : m1[q]) >> (jl & 31)) & 1u) != 0u;
+                            fs = ex ? 0.0f : fs;
+                            pair_add(dx, dy, dz, fs, lx[q], ly[q], lz[q]);
+                            if (ENERGY && act[q] && !ex) {
+                                en += lj_energy(inv, pr.x, pr.y);
+                                pair_virial(r2, fs, vir);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PPL; q++)
+                    if (act[q]) store(w_first + t, q, lx[q], ly[q], lz[q]);
+            }
+        } else if (valid) {
+            // 256 entries of the compound's solvent list (Engine.cu:967), staged in two halves
+            const int j0 = (k0 - ncc) * kSolventChunk;
+            const int nsrc = min(kSolventChunk, a.cs_cnt[c] - j0);
+            const int* ids = a.cs_ids + (size_t)c * a.cap_cs + j0;
+            float2 pr_cs[PPL];
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+                const float4 pv = a.tab_cs[r * kMaxTypes + tp[q]];
+                pr_cs[q] = lj_ab(pv.x, pv.y);
+            }
+            float lx[PPL], ly[PPL], lz[PPL];
+#pragma unroll
+            for (int q = 0; q < PPL; q++) lx[q] = ly[q] = lz[q] = 0.f;
+            for (int h0 = 0; h0 < nsrc; h0 += kPackRow) {
+                const int hn = min(kPackRow, nsrc - h0);
+                if (h0 > 0) wave_sync();   // the first half is summed before the row is rewritten
+#pragma unroll
+                for (int u = 0; u < kPackRow / kPackLanes; u++) {
+                    const int jj = pi + u * kPackLanes;
+                    if (jj < hn) {
+                        float4 qv = a.spos[ids[h0 + jj]];
+                        hyperpos(p0.x, p0.y, p0.z, qv.x, qv.y, qv.z, L, half);
+                        sg[jj] = qv;
+                    }
+                }
+                wave_sync();
+#pragma unroll 2
+                for (int j = 0; j < hn; j++) {
+                    const float4 qv = sg[j];
+#pragma unroll
+                    for (int q = 0; q < PPL; q++) {
+                        if (ENERGY) {
+                            double e0 = 0.0, w0 = 0.0;
+                            lj_acc_energy(mx[q], my[q], mz[q], qv.x, qv.y, qv.z, pr_cs[q].x, pr_cs[q].y, lx[q], ly[q],
+                                          lz[q], e0, w0);
+                            if (act[q]) { en += e0; vir += w0; }
+                        } else lj_acc(mx[q], my[q], mz[q], qv.x, qv.y, qv.z, pr_cs[q].x, pr_cs[q].y, lx[q], ly[q], lz[q]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PPL; q++)
+                if (act[q]) store(w_first, q, lx[q], ly[q], lz[q]);
         }
         if (ENERGY) {
-            en = group_sum<NPAD>(en);
-            vir = group_sum<NPAD>(vir);
+            en = group_sum<kPackLanes>(en);
+            vir = group_sum<kPackLanes>(vir);
             if (valid && pi == 0) {
                 if (en != 0.0) atomicAdd(a.energy + r * kEnergyTerms + (is_cc ? kE_inter : kE_cs), en);
                 if (vir != 0.0) atomicAdd(a.energy + r * kEnergyTerms + kE_virial, vir);
@@ -1361,6 +1440,8 @@ struct NlistArgs {
     int max_segs;
     int* cw_off; int* cw_total;
     int* item_comp;
+    int run_len;                  // neighbour items per run of the packed listed kernel (0: not used)
+    int* run_off; int* run_cs_off; int* run_total; int* run_comp;   // run_total[2]: neighbour runs, solvent chunks
     uint32_t* err;
     int enable_solvents;
 };
@@ -1718,39 +1799,62 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
     }
 }
 
-// per compound work slots: listed neighbours + ceil(solvent list / 256); exclusive scan
-__global__ __launch_bounds__(1024) void nl_work_kernel(NlistArgs a) {
-    __shared__ int part[1024];
+// per compound work slots: listed neighbours + ceil(solvent list / 256); exclusive scan.  With
+// run_len > 0 (the packed listed kernel), also the compound's runs: ceil(neighbours / run_len)
+// + ceil(solvent list / 256), scanned into run_off.
+// run_len > 0: the compound's runs of run_len neighbours; run_len == 0: its work items (one per
+// listed neighbour, then one per solvent chunk); run_len < 0: its solvent chunks alone
+LIMA_DEV int work_count(const NlistArgs& a, int c, int run_len) {
+    const int chunks = a.enable_solvents ? (a.cs_cnt[c] + kSolventChunk - 1) / kSolventChunk : 0;
+    if (run_len > 0) return (a.cc_cnt[c] + run_len - 1) / run_len;
+    if (run_len < 0) return chunks;
+    return a.cc_cnt[c] + chunks;
+}
+
+LIMA_DEV void block_exclusive_scan(int* part, const NlistArgs& a, int run_len, int* off, int* total) {
     const int n = a.n_comp;
     const int per = (n + 1023) / 1024;
     const int b = threadIdx.x * per;
-    auto slots = [&](int c) {
-        int s = a.cc_cnt[c];
-        if (a.enable_solvents) s += (a.cs_cnt[c] + kSolventChunk - 1) / kSolventChunk;
-        return s;
-    };
     int s = 0;
-    for (int k = 0; k < per && b + k < n; k++) s += slots(b + k);
+    for (int k = 0; k < per && b + k < n; k++) s += work_count(a, b + k, run_len);
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
         __syncthreads();
         part[threadIdx.x] += v;
         __syncthreads();
     }
     int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
     for (int k = 0; k < per && b + k < n; k++) {
-        a.cw_off[b + k] = run;
-        run += slots(b + k);
+        off[b + k] = run;
+        run += work_count(a, b + k, run_len);
     }
-    if (threadIdx.x == 1023) { a.cw_off[n] = part[1023]; *a.cw_total = part[1023]; }
+    if (threadIdx.x == 1023) { off[n] = part[1023]; *total = part[1023]; }
+    __syncthreads();
 }
 
-// the compound of every work item (one block per compound)
+// With run_len > 0 (the packed listed kernel) the run list holds every compound's neighbour
+// runs first (run_off, run_total[0] of them), then every compound's solvent chunks (run_cs_off,
+// run_total[1]), so that the groups of a wavefront mostly take runs of one kind.
+__global__ __launch_bounds__(1024) void nl_work_kernel(NlistArgs a) {
+    __shared__ int part[1024];
+    block_exclusive_scan(part, a, 0, a.cw_off, a.cw_total);
+    if (a.run_len > 0) {
+        block_exclusive_scan(part, a, a.run_len, a.run_off, a.run_total);
+        block_exclusive_scan(part, a, -1, a.run_cs_off, a.run_total + 1);
+    }
+}
+
+// the compound of every work item, and of every run (one block per compound)
 __global__ __launch_bounds__(64) void nl_item_kernel(NlistArgs a) {
     const int c = blockIdx.x;
     for (int w = a.cw_off[c] + threadIdx.x; w < a.cw_off[c + 1]; w += blockDim.x) a.item_comp[w] = c;
+    if (a.run_len > 0) {
+        for (int w = a.run_off[c] + threadIdx.x; w < a.run_off[c + 1]; w += blockDim.x) a.run_comp[w] = c;
+        const int cs0 = a.run_total[0];
+        for (int w = a.run_cs_off[c] + threadIdx.x; w < a.run_cs_off[c + 1]; w += blockDim.x) a.run_comp[cs0 + w] = c;
+    }
 }
 
 }  // namespace lima
