@@ -102,6 +102,8 @@ struct LaunchConfig {
     int listed_grid = 0;          // grid of compound_listed_kernel (0 = auto)
     int use_graph = 1;
     int listed_lanes = 1;         // lanes per solvent of the listed solvent kernel (1, or 4 in FP32)
+    int listed_packed = 1;        // compounds of <= 64 particles: the packed listed kernel (1), the plain one (0),
+                                  // or (FP32) the plain one with each small item's sources split over both waves (2)
 };
 
 // Device buffers of lima_adjoint_run / lima_record_frames, grown on demand and
@@ -322,6 +324,7 @@ struct lima_engine {
         a.enable_solvents = params.enable_solvents;
         a.enable_dihedrals = params.enable_dihedrals;
         a.loc_nb = max_cb; a.loc_na = max_ca; a.loc_nd = max_cd;
+        a.listed_split = (cfg.listed_packed == 2 && params.accumulation != LIMA_ACCUM_FIXED64) ? 1 : 0;
         return a;
     }
 
@@ -418,10 +421,12 @@ struct lima_engine {
         scwork.upload(scwork_h);
     }
 
-    // compound_local_kernel: one wavefront per compound when every compound fits one, else two;
-    // term results in dynamic LDS sized to the largest compound
+    // compound_local_kernel: one wavefront per compound when every compound fits one and there are
+    // enough compounds to fill the device (512, two per SIMD pair), else two (small systems run
+    // each compound's terms over twice the threads); term results in dynamic LDS sized to the
+    // largest compound
     template <bool FIXED, bool ENERGY> void launch_compound_local(const StepArgs& a, hipStream_t st) {
-        if (max_cn <= 64)
+        if (max_cn <= 64 && NC >= 512)
             compound_local_kernel<FIXED, ENERGY, 64><<<NC, 64, local_lds_bytes<64>(max_cb, max_ca, max_cd), st>>>(a);
         else
             compound_local_kernel<FIXED, ENERGY, 128><<<NC, 128, local_lds_bytes<128>(max_cb, max_ca, max_cd), st>>>(a);
@@ -433,7 +438,7 @@ struct lima_engine {
     // compound_listed_kernel, or its packed form when every compound fits 16 lanes x 1, 2 or 4 particles
     template <bool FIXED, bool ENERGY> void launch_compound_listed(const StepArgs& a, int grid, hipStream_t st) {
         // one-wavefront blocks: twice the blocks of the two-wavefront plain kernel's grid
-        switch (pack_ppl()) {
+        switch (cfg.listed_packed == 1 ? pack_ppl() : 0) {
             case 1: compound_listed_packed_kernel<FIXED, ENERGY, 1><<<2 * grid, 64, 0, st>>>(a); break;
             case 2: compound_listed_packed_kernel<FIXED, ENERGY, 2><<<2 * grid, 64, 0, st>>>(a); break;
             case 4: compound_listed_packed_kernel<FIXED, ENERGY, 4><<<2 * grid, 64, 0, st>>>(a); break;
@@ -1665,18 +1670,19 @@ void load_tune_file() {
     std::ifstream f(path);
     std::string line;
     while (std::getline(f, line)) {
-        // key spt chunk_particles listed_grid [listed_lanes]
+        // key spt chunk_particles listed_grid [listed_lanes [listed_packed]]
         std::istringstream is(line);
         std::string key;
         LaunchConfig c;
         if (!(is >> key >> c.spt >> c.chunk_particles >> c.listed_grid)) continue;
         if (!(is >> c.listed_lanes)) c.listed_lanes = 1;
+        if (!(is >> c.listed_packed)) c.listed_packed = 1;
         // only shapes the autotuner itself can pick: any other line is skipped
         const bool ok = (c.spt == 1 || c.spt == 2 || c.spt == 4) &&
                         (c.chunk_particles == 256 || c.chunk_particles == 512 || c.chunk_particles == 1024 ||
                          c.chunk_particles == 2048) && c.chunk_particles <= kChunkParticles &&
                         (c.listed_grid == 1024 || c.listed_grid == 2048 || c.listed_grid == 4096 || c.listed_grid == 8192) &&
-                        (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes);
+                        (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes) && (c.listed_packed == 0 || c.listed_packed == 1 || c.listed_packed == 2);
         if (!ok) continue;
         g_tune_cache[key] = c;
     }
@@ -1686,7 +1692,8 @@ void save_tune_entry(const std::string& key, const LaunchConfig& c) {
     const char* path = std::getenv("LIMA_AUTOTUNE_CACHE");
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << " " << c.listed_lanes << "\n";
+    f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << " " << c.listed_lanes << " "
+      << c.listed_packed << "\n";
 }
 
 // one system's energy record from the device sums (half of every directed-pair LJ sum)
@@ -2226,19 +2233,15 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
     });
 }
 
-int lima_get_config(lima_engine* E, int32_t* out10) {
+int lima_get_config(lima_engine* E, int32_t* out, int32_t n_out) {
     return guarded(E, [&] {
-        if (!out10) throw LimaError(LIMA_E_ARG, "null output");
-        out10[0] = E->cfg.spt;
-        out10[1] = E->cfg.chunk_particles;
-        out10[2] = (int)E->scwork_h.size();
-        out10[3] = E->kmax;
-        out10[4] = E->cfg.use_graph;
-        out10[5] = E->n_cells;
-        out10[6] = E->n_stiles;
-        out10[7] = E->R;
-        out10[8] = E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096;
-        out10[9] = E->params.accumulation == LIMA_ACCUM_FIXED64 ? 1 : E->cfg.listed_lanes;
+        if (!out || n_out < 0) throw LimaError(LIMA_E_ARG, "null output or negative count");
+        const int32_t v[11] = {E->cfg.spt, E->cfg.chunk_particles, (int32_t)E->scwork_h.size(), E->kmax, E->cfg.use_graph,
+                               E->n_cells, E->n_stiles, E->R, E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096,
+                               E->params.accumulation == LIMA_ACCUM_FIXED64 ? 1 : E->cfg.listed_lanes,
+                               !E->pack_ppl() ? 0 : E->cfg.listed_packed == 2 && E->params.accumulation != LIMA_ACCUM_FIXED64 ? 2
+                                                  : E->cfg.listed_packed == 1 ? 1 : 0};
+        for (int i = 0; i < std::min<int32_t>(n_out, 11); i++) out[i] = v[i];
         return LIMA_OK;
     });
 }
@@ -2358,7 +2361,10 @@ int lima_autotune(lima_engine* E, int32_t reps) {
             }
             // three sweeps, the fastest time of each shape (clocks ramp after idle periods)
             constexpr int kSweeps = 3;
-            if (E->params.enable_solvents && E->NS > 0 && E->NCH > 0) {
+            // the all-pairs shape, swept again after the listed kernels' choices below (a shape
+            // picked while a slow listed form dominates the step can be the wrong one once it is gone)
+            auto sweep_all_pairs = [&] {
+                if (!(E->params.enable_solvents && E->NS > 0 && E->NCH > 0)) return;
                 const int spts[3] = {1, 2, 4}, chunks[4] = {256, 512, 1024, 2048};
                 double tmin[3][4];
                 for (auto& row : tmin)
@@ -2376,22 +2382,62 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 for (int i = 0; i < 3; i++)
                     for (int j = 0; j < 4; j++)
                         if (tmin[i][j] < tbest) { tbest = tmin[i][j]; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
+                // shapes within 1% of the best step are ties at the trials' resolution: among them
+                // the one whose all-pairs kernel runs fastest alone (the others hide a slower
+                // kernel behind the concurrent streams at this size, and may not at another)
+                double kbest = 1e30;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 4; j++) {
+                        if (tmin[i][j] > 1.01 * tbest) continue;
+                        E->cfg.spt = spts[i];
+                        E->cfg.chunk_particles = chunks[j];
+                        E->build_scwork();
+                        E->reps.upload(E->reps_h);
+                        StepArgs ka = E->step_args(E->cur);
+                        ka.kclock = nullptr;
+                        auto launch = [&] {
+                            if (E->params.accumulation == LIMA_ACCUM_FIXED64) E->launch_solvent_chain<true, false>(ka, E->stream);
+                            else E->launch_solvent_chain<false, false>(ka, E->stream);
+                        };
+                        launch();
+                        HIP_TRY(hipEventRecord(e0, E->stream));
+                        for (int k = 0; k < 5; k++) launch();
+                        HIP_TRY(hipEventRecord(e1, E->stream));
+                        HIP_TRY(hipEventSynchronize(e1));
+                        float ms = 0;
+                        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                        if (ms < kbest) { kbest = ms; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
+                    }
+                restore();
                 E->cfg.spt = best.spt;
                 E->cfg.chunk_particles = best.chunk_particles;
-            }
+            };
+            sweep_all_pairs();
             if (E->NC > 0) {
+                // grid sizes, with the packed and the plain listed kernel where compounds allow both
+                // (they give the same sums: a small system fills the device better with the plain one)
+                // forms: packed (1), plain (0), plain with split small items (2, FP32 only)
                 const int grids[4] = {1024, 2048, 4096, 8192};
-                double gmin[4] = {1e30, 1e30, 1e30, 1e30};
+                const int forms[3] = {1, 0, 2};
+                const int nform = !E->pack_ppl() ? 1 : (E->params.accumulation == LIMA_ACCUM_FIXED64 ? 2 : 3);
+                double gmin[3][4];
+                for (auto& row : gmin)
+                    for (double& v : row) v = 1e30;
                 for (int sweep = 0; sweep < kSweeps; sweep++) {
-                    for (int j = 0; j < 4; j++) {
-                        E->cfg.listed_grid = grids[j];
-                        gmin[j] = std::min(gmin[j], trial());
+                    for (int k = 0; k < nform; k++) {
+                        E->cfg.listed_packed = forms[k];
+                        for (int j = 0; j < 4; j++) {
+                            E->cfg.listed_grid = grids[j];
+                            gmin[k][j] = std::min(gmin[k][j], trial());
+                        }
                     }
                 }
                 double gbest = 1e30;
-                for (int j = 0; j < 4; j++)
-                    if (gmin[j] < gbest) { gbest = gmin[j]; best.listed_grid = grids[j]; }
+                for (int k = 0; k < nform; k++)
+                    for (int j = 0; j < 4; j++)
+                        if (gmin[k][j] < gbest) { gbest = gmin[k][j]; best.listed_grid = grids[j]; best.listed_packed = forms[k]; }
                 E->cfg.listed_grid = best.listed_grid;
+                E->cfg.listed_packed = best.listed_packed;
             }
             if (E->params.enable_solvents && E->NS > 0 && E->params.accumulation != LIMA_ACCUM_FIXED64) {
                 // one or four lanes per solvent in the listed solvent kernel (FP32 only: the four
@@ -2405,7 +2451,9 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                     }
                 }
                 best.listed_lanes = lmin[1] < lmin[0] ? kSplitLanes : 1;
+                E->cfg.listed_lanes = best.listed_lanes;
             }
+            sweep_all_pairs();
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             std::lock_guard<std::mutex> lk(g_tune_mu);

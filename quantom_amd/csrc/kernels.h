@@ -102,6 +102,7 @@ struct StepArgs {
     uint32_t* err;                // [4]
     int enable_solvents, enable_dihedrals;
     int loc_nb, loc_na, loc_nd;   // most bonds, angles, dihedrals of any compound (compound_local LDS strides)
+    int listed_split;             // FP32: compound_listed_kernel splits the sources of compounds of <= 64 particles over both waves
 };
 
 enum { kE_bond = 0, kE_angle, kE_dih, kE_intra, kE_inter, kE_cs, kE_sc, kE_ss, kE_virial, kE_kin, kEnergyTerms };
@@ -376,6 +377,7 @@ template <bool FIXED, bool ENERGY>
 __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
     __shared__ float4 src[kSolventChunk];
     __shared__ float2 tab[kMaxTypes * kMaxTypes];   // (A, B) of lj_ab: 8-byte entries keep the per-lane reads conflict-light
+    __shared__ float4 xch[kCompoundBlock / 2];      // the second wave's partial sums of a split item
     __shared__ int cached_rep;
     const int p = threadIdx.x;
     if (p == 0) cached_rep = -1;
@@ -387,9 +389,13 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
         const ReplicaInfo rp = a.reps[r];
         const float L = rp.L, half = rp.half;
         const int base = a.comp_off[c], n = a.comp_off[c + 1] - base;
-        // one thread per particle sums the item's sources whole, in list order (as the packed
-        // form does, so that a replica gives the same sums in any batch)
-        const int pi = p;
+        // one thread per particle sums the item's sources whole, in list order, as the packed form
+        // does.  With a.listed_split (FP32 only, an autotuner choice for small systems) a compound
+        // of at most one wavefront's particles has both waves take particle pi, each one contiguous
+        // half of the item's sources, the halves added in order at the end.
+        const bool split = !FIXED && a.listed_split && n <= kCompoundBlock / 2;
+        const int pi = split ? (p & (kCompoundBlock / 2 - 1)) : p;
+        const int hs = split ? (p / (kCompoundBlock / 2)) : 0;
         const bool active = pi < n;
         float4 me = a.cpos[base + (active ? pi : 0)];
         const float4 p0 = a.cpos[base];
@@ -434,7 +440,8 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             // exclusions across compounds exist only between bridged compounds: a wave
             // with none against this neighbour takes the loop without the mask test
             const bool wave_clean = __ballot((m0 | m1 | m2 | m3) != 0u) == 0ull;
-            const int jlo = 0, jhi = nn;
+            const int hn = split ? (nn + 1) / 2 : nn;
+            const int jlo = hs * hn, jhi = min(nn, jlo + hn);
             if (active && wave_clean) {
                 const float2* trow = tab + tp;
 #pragma unroll 4
@@ -492,7 +499,8 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
             __syncthreads();
             const float4 pv = a.tab_cs[r * kMaxTypes + tp];
             const float2 pr = lj_ab(pv.x, pv.y);
-            const int jlo = 0, jhi = nsrc;
+            const int hn = split ? (nsrc + 1) / 2 : nsrc;
+            const int jlo = hs * hn, jhi = min(nsrc, jlo + hn);
             if (active) {
                 for (int j = jlo; j < jhi; j++) {
                     const float4 q = src[j];
@@ -505,7 +513,12 @@ __global__ __launch_bounds__(128) void compound_listed_kernel(StepArgs a) {
                 block_atomic_energy(vir, a.energy + r * kEnergyTerms + kE_virial);
             }
         }
-        if (active) {
+        if (split) {
+            if (hs == 1) xch[pi] = make_float4(lx, ly, lz, 0.f);
+            __syncthreads();
+            if (hs == 0 && active) a.partC[(size_t)w * kCompoundBlock + pi] = make_float4(lx + xch[pi].x, ly + xch[pi].y,
+                                                                                     lz + xch[pi].z, 0.f);
+        } else if (active) {
             const size_t slot = (size_t)w * kCompoundBlock + p;
             if (FIXED) {
                 uint32_t bad = 0;

@@ -71,7 +71,7 @@ def load_library() -> C.CDLL:
     L.lima_get_neighbors.argtypes = [vp, ip, ip, ip]
     L.lima_set_neighbors.argtypes = [vp, ip, ip, ip, ip, ip, ip]
     L.lima_get_timings.argtypes = [vp, C.POINTER(C.c_double)]
-    L.lima_get_config.argtypes = [vp, ip]
+    L.lima_get_config.argtypes = [vp, ip, C.c_int32]
     L.lima_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
     L.lima_set_kernel_clock.argtypes = [vp, C.c_int32]
     L.lima_get_kernel_clock.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -306,10 +306,10 @@ class Engine:
         return us.value, n.value
 
     def config(self) -> dict:
-        c = (C.c_int32 * 10)()
-        self._check(self.L.lima_get_config(self.h, c))
+        c = (C.c_int32 * 11)()
+        self._check(self.L.lima_get_config(self.h, c, 11))
         keys = ("solvents_per_thread", "chunk_particles", "solvent_chain_items", "max_chunks", "use_graph",
-                "n_cells", "n_solvent_tiles", "n_replicas", "listed_grid", "listed_lanes")
+                "n_cells", "n_solvent_tiles", "n_replicas", "listed_grid", "listed_lanes", "listed_packed")
         return dict(zip(keys, list(c)))
 
     def autotune(self, trial_steps: int = 20) -> dict:

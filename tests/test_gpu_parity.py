@@ -369,3 +369,38 @@ def test_four_lanes_per_solvent_listed_form(small_system, monkeypatch, tmp_path)
     assert d4 <= 3 * d1 + 1e-5
     four.close()
     one.close()
+
+
+def test_listed_compound_forms(tiny_system, monkeypatch, tmp_path):
+    """Compounds of at most 64 particles: the packed listed kernel (the default), the plain one and,
+    in FP32, the plain one with each item's sources split over both wavefronts (autotuner choices).
+    Every form gives forces within the oracle bar; packed and plain give bit-equal sums (one
+    sequential sum per particle and item); int64 accumulation never splits, so a requested split
+    form steps bit-equal to the packed default."""
+    from oracle.oracle import Oracle
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print("case: listed compound forms, tiny system", flush=True)
+    s = tiny_system
+    fo = np.concatenate([x.ravel() for x in Oracle(s, RunParams(nlist_interval=10), precision=32).compute_forces()[:2]])
+    forces = {}
+    for acc in (0, 1):
+        key = f"acc{acc}_sol1_{(s.n_chain + 64) // 128}x{(s.n_solvents + 256) // 512}:1"
+        for form in (1, 0, 2):
+            # the library reads a cache file once per path: one file per form
+            path = tmp_path / f"tune_{acc}_{form}.txt"
+            path.write_text(f"{key} 2 1024 4096 1 {form}\n")
+            monkeypatch.setenv("LIMA_AUTOTUNE_CACHE", str(path))
+            e = Engine([s], RunParams(nlist_interval=10, accumulation=acc))
+            cfg = e.autotune(10)
+            assert cfg["listed_packed"] == (form if not (acc == 1 and form == 2) else 0), (acc, form, cfg)
+            f = np.concatenate([x.ravel() for x in e.compute_forces()])
+            e.step(15)
+            forces[(acc, form)] = (f, e.get_state())
+            e.close()
+            print(f"accumulation {acc} form {form}: rel rms vs oracle {rel_rms(f, fo):.2e}", flush=True)
+            assert rel_rms(f, fo) <= 1e-5
+    for acc in (0, 1):
+        np.testing.assert_array_equal(forces[(acc, 1)][0], forces[(acc, 0)][0])
+    for x, y in zip(forces[(1, 1)][1], forces[(1, 2)][1]):
+        np.testing.assert_array_equal(x, y)
