@@ -1699,7 +1699,6 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
         const int z = (n >= 3) ? (cz - 1 + oz + n) % n : oz;
         return rp.cell_begin + (x * n + y) * n + z;
     };
-    const int k_home = (n >= 3) ? 13 : (cx * span + cy) * span + cz;
     // all cell ranges at once, then an exclusive scan of their sizes
     if (threadIdx.x < nnb) {
         const int c = nb_cell(threadIdx.x);
@@ -1718,40 +1717,67 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
     }
     __syncthreads();
     const int total = cell_off[nnb];
-    if (total <= kCellCandidates) {
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        // stage the candidates in the fixed cell order: wavefront w copies cells w, w + nw, ...
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // With 3 or more cells per side each neighbour cell has one image next to the home cell, and
+    // its candidates are staged shifted into it, so that the prefilter below is a plain distance
+    // (the exact test reads the raw positions); narrower grids stage them raw.
+    const bool shifted = n >= 3;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int nhome = h1 - h0;
+    const float rb = sqrtf(bound) + 1e-5f * L + 1e-6f;
+    const float pre = rb * rb;
+    unsigned short* q = queue[wid];
+    // The candidates, in the fixed cell order, pass through LDS kCellCandidates at a time; each
+    // home solvent takes every chunk in turn, so its row is in candidate order whatever the chunks.
+    // Between chunks the row's running count (not yet clamped to the capacity) waits in ss_cnt.
+    for (int c0 = 0; c0 < total; c0 += kCellCandidates) {
+        const int c1 = min(total, c0 + kCellCandidates);
+        const bool last = c1 == total;
+        if (c0 > 0) {
+            // every wavefront is done with the previous chunk, and the running counts it stored
+            // are visible to the loads below
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        // stage: wavefront w copies the parts of cells w, w + nw, ... that fall in [c0, c1)
         for (int k = wid; k < nnb; k += nw) {
             const int b = cell_b[k], o = cell_off[k], m = cell_n[k];
-            for (int e = lane; e < m; e += 64) {
+            const int e0 = max(0, c0 - o), e1 = min(m, c1 - o);
+            if (e0 >= e1) continue;
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            if (shifted) {
+                const int ux = cx - 1 + k / 9, uy = cy - 1 + (k / 3) % 3, uz = cz - 1 + k % 3;
+                sx = ux < 0 ? -L : ux >= n ? L : 0.f;
+                sy = uy < 0 ? -L : uy >= n ? L : 0.f;
+                sz = uz < 0 ? -L : uz >= n ? L : 0.f;
+            }
+            for (int e = e0 + lane; e < e1; e += 64) {
                 const int id = a.cell_ids[b + e];
-                cand_id[o + e] = id;
-                cand_pos[o + e] = a.spos[id];
+                const float4 x = a.spos[id];
+                cand_id[o + e - c0] = id;
+                cand_pos[o + e - c0] = make_float4(x.x + sx, x.y + sy, x.z + sz, x.w);
             }
         }
         // pad to whole passes of 128 with NaN positions, which no distance test accepts
-        const int padded = (total + 127) & ~127;
-        if (threadIdx.x < padded - total) {
+        const int nc = c1 - c0;
+        const int padded = (nc + 127) & ~127;
+        if (threadIdx.x < padded - nc) {
             const float nan = __builtin_nanf("");
-            cand_pos[total + threadIdx.x] = make_float4(nan, nan, nan, nan);
+            cand_pos[nc + threadIdx.x] = make_float4(nan, nan, nan, nan);
         }
         __syncthreads();
-        // Each wavefront takes home solvents (staged with the home cell) in turn.  A cheap
-        // minimum-image distance, with a margin far above its rounding error against the exact
+        // Each wavefront takes home solvents in turn.  A cheap distance (minimum image on grids
+        // narrower than 3 cells), with a margin far above its rounding error against the exact
         // re-imaged one, drops the candidates that cannot be members; the rest enter the
         // wavefront's queue in candidate order.  Whenever 64 are queued, the exact predicate
-        // runs on them and the members are appended in that order (ballot compaction), so
-        // each list is the one a test of every candidate in order would give.
-        const unsigned long long below = (1ull << lane) - 1ull;
-        const int home0 = cell_off[k_home], nhome = h1 - h0;
-        const float rb = sqrtf(bound) + 1e-5f * L + 1e-6f;
-        const float pre = rb * rb;
-        unsigned short* q = queue[wid];
+        // runs on them and the members are appended in that order (ballot compaction), so each
+        // list is the one a test of every candidate in order would give.
         for (int h = wid; h < nhome; h += nw) {
-            const int s = cand_id[home0 + h];
-            const float4 me = cand_pos[home0 + h];
+            const int s = a.cell_ids[h0 + h];
+            const float4 me = a.spos[s];
             int* row = a.ss_ids + (size_t)s * a.ss_ld;
-            int count = 0;
+            int count = c0 == 0 ? 0 : a.ss_cnt[s];
             unsigned head = 0, tail = 0;   // wavefront-uniform queue cursors
             auto drain = [&](int m) {      // the exact test on the m <= 64 oldest queued slots
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // other lanes' queue writes
@@ -1760,7 +1786,7 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
                 const bool has = lane < m;
                 const int e = has ? (int)q[(head + lane) & (kSsQueue - 1)] : 0;
                 const int j = cand_id[e];
-                const bool mem = has & ss_member(s, me, j, cand_pos[e], L, half, bound);
+                const bool mem = has & ss_member(s, me, j, shifted ? a.spos[j] : cand_pos[e], L, half, bound);
                 const unsigned long long bm = __ballot(mem);
                 const int pos = count + __popcll(bm & below);
                 if (mem & (pos < a.cap_ss)) row[pos] = j;
@@ -1769,8 +1795,11 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
             };
             auto near = [&](int e) {
                 const float4 c = cand_pos[e];
-                float dx = fabsf(c.x - me.x), dy = fabsf(c.y - me.y), dz = fabsf(c.z - me.z);
-                dx = fminf(dx, fabsf(L - dx)); dy = fminf(dy, fabsf(L - dy)); dz = fminf(dz, fabsf(L - dz));
+                float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+                if (!shifted) {
+                    dx = fabsf(dx); dy = fabsf(dy); dz = fabsf(dz);
+                    dx = fminf(dx, fabsf(L - dx)); dy = fminf(dy, fabsf(L - dy)); dz = fminf(dz, fabsf(L - dz));
+                }
                 return dx * dx + dy * dy + dz * dz < pre;
             };
             for (int e0 = 0; e0 < padded; e0 += 128) {
@@ -1785,30 +1814,14 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
             }
             while (tail != head) drain((int)min(64u, tail - head));
             if (lane == 0) {
-                if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
-                a.ss_cnt[s] = min(count, a.cap_ss);
-            }
-        }
-        return;
-    }
-    for (int h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
-        const int s = a.cell_ids[h];
-        const float4 me = a.spos[s];
-        int count = 0;
-        {   // more candidates than LDS holds: same order, read from global memory
-            for (int k = 0; k < nnb; k++) {
-                const int b = cell_b[k], e1 = b + cell_n[k];
-                for (int e = b; e < e1; e++) {
-                    const int j = a.cell_ids[e];
-                    if (ss_member(s, me, j, a.spos[j], L, half, bound)) {
-                        if (count < a.cap_ss) a.ss_ids[(size_t)s * a.ss_ld + count] = j;
-                        count++;
-                    }
+                if (!last) {
+                    a.ss_cnt[s] = count;
+                } else {
+                    if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
+                    a.ss_cnt[s] = min(count, a.cap_ss);
                 }
             }
         }
-        if (count > a.cap_ss) { atomicOr(a.err, kErrCapacity); atomicMax(a.err + 3, count); }
-        a.ss_cnt[s] = min(count, a.cap_ss);
     }
 }
 

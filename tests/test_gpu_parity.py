@@ -258,6 +258,32 @@ def test_solvent_lists_at_the_bound_and_across_faces():
     eng.close()
 
 
+@pytest.mark.parametrize("box,n_solvents", [(4.7, 3700), (3.5, 3700)])
+def test_solvent_lists_beyond_one_lds_stage(box, n_solvents):
+    """Home cells whose 27-cell neighbourhood holds more candidates than one LDS stage (3,456) take
+    the candidates in chunks, each home solvent every chunk in turn: lists as the oracle's.  4.7 nm:
+    3 cells per side (candidates staged shifted into the home cell's image); 3.5 nm: 2 cells per
+    side (minimum-image prefilter).  Either way every cell's neighbourhood is the whole box."""
+    from oracle.oracle import KIND_SS, Oracle
+    from quantom_amd import systems
+    from quantom_amd.abi import RunParams
+    from quantom_amd.engine import Engine
+    print(f"case: solvent lists, {n_solvents} solvents in a {box} nm box", flush=True)
+    s = systems.build_system(0x51A7 + 93, 1, 1, box, 4, n_solvents, compound_size=16, auto_density=True)
+    assert s.n_solvents == n_solvents
+    p = RunParams(max_solvent_neighbors=2048)
+    want = Oracle(s, p, precision=32).neighbors(KIND_SS)
+    eng = Engine([s], p)
+    got = eng.neighbors()[2]
+    diff = [(i, sorted(set(g) ^ set(w))) for i, (g, w) in enumerate(zip(got, want)) if set(g) != set(w)]
+    for i, d in diff[:8]:
+        print(f"solvent {i}: differs in {d}", flush=True)
+    assert not diff
+    assert all(len(g) == len(set(g)) for g in got)
+    print(f"mean list length {np.mean([len(g) for g in got]):.1f}", flush=True)
+    eng.close()
+
+
 def test_compound_solvent_lists_at_the_bound():
     """Solvents placed within a few ulps of a compound's list radius: C + r + C from its key
     particle, r the confining sphere (Engine.cu:146-151, 188-200), in random directions and
