@@ -1588,16 +1588,41 @@ __global__ void nl_cell_zero_kernel(int* cell_cnt, int n) {
     if (i < n) cell_cnt[i] = 0;
 }
 
+// Wavefront-aggregated counter update: the lanes of each distinct cell in the wavefront add their
+// number with one atomic, made by the cell's first lane; returns each lane's slot (the counter's
+// old value plus the lane's rank among the cell's lanes).  Consecutive solvents mostly share a
+// few cells, so the atomics on one counter drop from one per solvent to one per wavefront.
+LIMA_DEV int wave_cell_add(int* counter, int cell, bool valid) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(valid);
+    int slot = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lc = __shfl(cell, leader);
+        const unsigned long long same = __ballot(valid && cell == lc) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(counter + lc, __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) slot = base + __popcll(same & below);
+        todo &= ~same;
+    }
+    return slot;
+}
+
 __global__ void nl_cell_count_kernel(NlistArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n_solv) return;
-    const ReplicaInfo rp = a.reps[a.solv_rep[s]];
-    const float4 q = a.spos[s];
-    const int n = rp.ncell;
-    const int cx = cell_coord(q.x, rp.cell_inv, n), cy = cell_coord(q.y, rp.cell_inv, n), cz = cell_coord(q.z, rp.cell_inv, n);
-    const int cell = rp.cell_begin + (cx * n + cy) * n + cz;
-    a.solv_cell[s] = cell;
-    atomicAdd(&a.cell_cnt[cell], 1);
+    const bool valid = s < a.n_solv;
+    int cell = 0;
+    if (valid) {
+        const ReplicaInfo rp = a.reps[a.solv_rep[s]];
+        const float4 q = a.spos[s];
+        const int n = rp.ncell;
+        const int cx = cell_coord(q.x, rp.cell_inv, n), cy = cell_coord(q.y, rp.cell_inv, n), cz = cell_coord(q.z, rp.cell_inv, n);
+        cell = rp.cell_begin + (cx * n + cy) * n + cz;
+        a.solv_cell[s] = cell;
+    }
+    wave_cell_add(a.cell_cnt, cell, valid);
 }
 
 // exclusive scan of cell counts (single block of 1024)
@@ -1627,9 +1652,10 @@ __global__ __launch_bounds__(1024) void nl_cell_scan_kernel(NlistArgs a) {
 
 __global__ void nl_cell_scatter_kernel(NlistArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n_solv) return;
-    const int pos = atomicAdd(&a.cell_cursor[a.solv_cell[s]], 1);
-    a.cell_ids[pos] = s;
+    const bool valid = s < a.n_solv;
+    const int cell = valid ? a.solv_cell[s] : 0;
+    const int pos = wave_cell_add(a.cell_cursor, cell, valid);
+    if (valid) a.cell_ids[pos] = s;
 }
 
 // sort each cell's ids ascending (rank sort in LDS), making the lists deterministic
@@ -1726,6 +1752,10 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
     const int nhome = h1 - h0;
     const float rb = sqrtf(bound) + 1e-5f * L + 1e-6f;
     const float pre = rb * rb;
+    // below this the cheap distance certifies membership: per axis it differs from the exactly
+    // rounded re-imaged difference by a few roundings of coordinates up to 2L, far below the margin
+    const float ri = sqrtf(bound) - (1e-5f * L + 1e-6f);
+    const float inner = ri * ri;
     unsigned short* q = queue[wid];
     // The candidates, in the fixed cell order, pass through LDS kCellCandidates at a time; each
     // home solvent takes every chunk in turn, so its row is in candidate order whatever the chunks.
@@ -1779,29 +1809,34 @@ __global__ __launch_bounds__(kSolventListThreads) void nl_solvent_solvent_kernel
             int* row = a.ss_ids + (size_t)s * a.ss_ld;
             int count = c0 == 0 ? 0 : a.ss_cnt[s];
             unsigned head = 0, tail = 0;   // wavefront-uniform queue cursors
-            auto drain = [&](int m) {      // the exact test on the m <= 64 oldest queued slots
+            auto dist2 = [&](float4 c) {   // the cheap distance of a staged candidate
+                float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+                if (!shifted) {
+                    dx = fabsf(dx); dy = fabsf(dy); dz = fabsf(dz);
+                    dx = fminf(dx, fabsf(L - dx)); dy = fminf(dy, fabsf(L - dy)); dz = fminf(dz, fabsf(L - dz));
+                }
+                return dx * dx + dy * dy + dz * dz;
+            };
+            auto drain = [&](int m) {      // membership of the m <= 64 oldest queued slots
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // other lanes' queue writes
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 const bool has = lane < m;
                 const int e = has ? (int)q[(head + lane) & (kSsQueue - 1)] : 0;
                 const int j = cand_id[e];
-                const bool mem = has & ss_member(s, me, j, shifted ? a.spos[j] : cand_pos[e], L, half, bound);
+                const float d2 = dist2(cand_pos[e]);
+                bool mem = has & (j != s) & (d2 < inner);
+                // the exact predicate only for slots near the bound (wavefront-uniform branch)
+                const bool band = has & !mem & (j != s);
+                if (__ballot(band) != 0ull)
+                    mem = mem | (band && ss_member(s, me, j, shifted ? a.spos[j] : cand_pos[e], L, half, bound));
                 const unsigned long long bm = __ballot(mem);
                 const int pos = count + __popcll(bm & below);
                 if (mem & (pos < a.cap_ss)) row[pos] = j;
                 count += __popcll(bm);
                 head += m;
             };
-            auto near = [&](int e) {
-                const float4 c = cand_pos[e];
-                float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-                if (!shifted) {
-                    dx = fabsf(dx); dy = fabsf(dy); dz = fabsf(dz);
-                    dx = fminf(dx, fabsf(L - dx)); dy = fminf(dy, fabsf(L - dy)); dz = fminf(dz, fabsf(L - dz));
-                }
-                return dx * dx + dy * dy + dz * dz < pre;
-            };
+            auto near = [&](int e) { return dist2(cand_pos[e]) < pre; };
             for (int e0 = 0; e0 < padded; e0 += 128) {
                 const int ea = e0 + lane, eb = ea + 64;
                 const bool ma = near(ea), mb = near(eb);
