@@ -177,8 +177,8 @@ def test_filter_type_resolution(tmp_path):
 def _system(**kw):
     from quantom_amd import ingest
     import tempfile
-    d = tempfile.mkdtemp(prefix="ingest_", dir=os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else None)
-    return ingest.ingest(GRO, TOP, PRM, d, **kw)
+    with tempfile.TemporaryDirectory(prefix="ingest_") as d:   # the force-field files written on the way
+        return ingest.ingest(GRO, TOP, PRM, d, **kw)
 
 
 def test_compounds_bridges_and_exclusions_from_topology():
