@@ -2382,7 +2382,8 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 for (int i = 0; i < 3; i++)
                     for (int j = 0; j < 4; j++)
                         if (tmin[i][j] < tbest) { tbest = tmin[i][j]; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
-                // shapes within 1% of the best step are ties at the trials' resolution: among them
+                // shapes within 2% of the best step are ties at the trials' resolution (repeated
+                // whole-step trials of one shape spread by about that much): among them
                 // the ones whose all-pairs kernel runs within 2% of the fastest alone (the others
                 // hide a slower kernel behind the concurrent streams at this size, and may not at
                 // another), and of those the one writing the fewest per-chunk partials (less HBM
@@ -2395,7 +2396,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                     for (int j = 0; j < 4; j++) {
                         kms[i][j] = 1e30;
                         parts[i][j] = 0;
-                        if (tmin[i][j] > 1.01 * tbest) continue;
+                        if (tmin[i][j] > 1.02 * tbest) continue;
                         E->cfg.spt = spts[i];
                         E->cfg.chunk_particles = chunks[j];
                         E->build_scwork();
