@@ -2383,12 +2383,13 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                     for (int j = 0; j < 4; j++)
                         if (tmin[i][j] < tbest) { tbest = tmin[i][j]; best.spt = spts[i]; best.chunk_particles = chunks[j]; }
                 // shapes within 2% of the best step are ties at the trials' resolution (repeated
-                // whole-step trials of one shape spread by about that much): among them
-                // the ones whose all-pairs kernel runs within 2% of the fastest alone (the others
-                // hide a slower kernel behind the concurrent streams at this size, and may not at
-                // another), and of those the one writing the fewest per-chunk partials (less HBM
-                // traffic, fewer blocks beside the other streams; isolated times of shapes that
-                // differ only in chunking agree to within the timing noise)
+                // whole-step trials of one shape spread by about that much).  Among them, those
+                // whose all-pairs kernel alone is within 5% of the fastest (a shape with too few
+                // work items to fill the device alone leaves the step to the concurrent streams
+                // at this size and may not at another), and of those the one writing the fewest
+                // per-chunk partials (less HBM traffic, fewer blocks beside the other streams;
+                // shapes that differ only in chunking run within a few per cent alone), then the
+                // fastest alone
                 double kms[3][4];
                 long long parts[3][4];
                 double kbest = 1e30;
@@ -2424,7 +2425,7 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 double kpick = 1e30;
                 for (int i = 0; i < 3; i++)
                     for (int j = 0; j < 4; j++) {
-                        if (kms[i][j] > 1.02 * kbest) continue;
+                        if (kms[i][j] > 1.05 * kbest) continue;   // not a step tie, or slow alone
                         if (pbest < 0 || parts[i][j] < pbest || (parts[i][j] == pbest && kms[i][j] < kpick)) {
                             pbest = parts[i][j];
                             kpick = kms[i][j];
