@@ -13,6 +13,7 @@
 // box length, time step and type tables of that replica, so one launch covers
 // any number of independent systems.
 #pragma once
+#include <type_traits>
 #include "bonded.h"
 #include "device_common.h"
 
@@ -609,6 +610,11 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                 if (bad) atomicOr(a.err, bad);
             } else a.partC[slot] = make_float4(lx, ly, lz, 0.f);
         };
+        // The pair loops, for the first NQ particle slots of each lane: when every group of the
+        // wavefront holds a compound of at most 16 particles the second slot is idle, and the
+        // loops skip it (a wavefront-uniform choice; the active slots' sums are unchanged)
+        auto pairs = [&](auto nq_tag) {
+        constexpr int NQ = decltype(nq_tag)::value;
         wave_sync();   // every group of the wavefront is done with the previous round's rows
         if (valid && is_cc) {
             // the run's neighbour compounds, re-imaged against this compound's particle 0 (Engine.cu:949)
@@ -640,7 +646,7 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                 uint32_t m0[PPL], m1[PPL];
                 bool clean = true;
 #pragma unroll
-                for (int q = 0; q < PPL; q++) {
+                for (int q = 0; q < NQ; q++) {
                     m0[q] = m1[q] = 0u;
                     for (int e = xb[q]; e < xe[q]; e++) {
                         const int2 x = a.xent[e];
@@ -654,14 +660,14 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                 }
                 float lx[PPL], ly[PPL], lz[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; q++) lx[q] = ly[q] = lz[q] = 0.f;
+                for (int q = 0; q < NQ; q++) lx[q] = ly[q] = lz[q] = 0.f;
                 if (clean) {
 #pragma unroll 2
                     for (int j = j0; j < j1; j++) {
                         const float4 qv = sg[j];
                         const int ts = ftype(qv) * kMaxTypes;
 #pragma unroll
-                        for (int q = 0; q < PPL; q++) {
+                        for (int q = 0; q < NQ; q++) {
                             const float2 pr = tab[ts + tp[q]];
                             const float dx = qv.x - mx[q], dy = qv.y - my[q], dz = qv.z - mz[q];
                             const float r2 = pair_r2(dx, dy, dz);
@@ -679,7 +685,7 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                         const float4 qv = sg[j];
                         const int ts = ftype(qv) * kMaxTypes, jl = j - j0;
 #pragma unroll
-                        for (int q = 0; q < PPL; q++) {
+                        for (int q = 0; q < NQ; q++) {
                             const float2 pr = tab[ts + tp[q]];
                             const float dx = qv.x - mx[q], dy = qv.y - my[q], dz = qv.z - mz[q];
                             const float r2 = pair_r2(dx, dy, dz);
@@ -696,7 +702,7 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < PPL; q++)
+                for (int q = 0; q < NQ; q++)
                     if (act[q]) store(w_first + t, q, lx[q], ly[q], lz[q]);
             }
         } else if (valid) {
@@ -712,7 +718,7 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
             }
             float lx[PPL], ly[PPL], lz[PPL];
 #pragma unroll
-            for (int q = 0; q < PPL; q++) lx[q] = ly[q] = lz[q] = 0.f;
+            for (int q = 0; q < NQ; q++) lx[q] = ly[q] = lz[q] = 0.f;
             for (int h0 = 0; h0 < nsrc; h0 += kPackRow) {
                 const int hn = min(kPackRow, nsrc - h0);
                 if (h0 > 0) wave_sync();   // the first half is summed before the row is rewritten
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                 for (int j = 0; j < hn; j++) {
                     const float4 qv = sg[j];
 #pragma unroll
-                    for (int q = 0; q < PPL; q++) {
+                    for (int q = 0; q < NQ; q++) {
                         if (ENERGY) {
                             double e0 = 0.0, w0 = 0.0;
                             lj_acc_energy(mx[q], my[q], mz[q], qv.x, qv.y, qv.z, pr_cs[q].x, pr_cs[q].y, lx[q], ly[q],
@@ -741,9 +747,12 @@ __global__ __launch_bounds__(64) void compound_listed_packed_kernel(StepArgs a) 
                 }
             }
 #pragma unroll
-            for (int q = 0; q < PPL; q++)
+            for (int q = 0; q < NQ; q++)
                 if (act[q]) store(w_first, q, lx[q], ly[q], lz[q]);
         }
+        };
+        if (PPL > 1 && __ballot(valid && n > kPackLanes) == 0ull) pairs(std::integral_constant<int, 1>{});
+        else pairs(std::integral_constant<int, PPL>{});
         if (ENERGY) {
             en = group_sum<kPackLanes>(en);
             vir = group_sum<kPackLanes>(vir);
