@@ -1134,7 +1134,6 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (k0 + CH < wmax) load_ids(k0 + CH);
         const int n = min(CH, cnt - k0);   // this lane's entries in the segment (may be <= 0)
         const int* mine = st + lane * STR;
         // all of the segment's neighbour positions in flight at once; slots past the row's
@@ -1148,12 +1147,44 @@ __global__ __launch_bounds__(64) void solvent_listed_kernel(StepArgs a, int n_it
             q[u + 2] = a.spos[u + 2 < n ? i4.z : si];
             q[u + 3] = a.spos[u + 3 < n ? i4.w : si];
         }
+        // The next segment's ids are requested after the positions.  The wait counter
+        // retires loads in issue order, so a wait for a position (an L2 hit) then does not
+        // also wait for the ids (HBM); they have the whole segment's sums to arrive.  The
+        // request is unconditional (past the last segment every row re-reads its entry 0):
+        // behind a branch, the waits after it would be counted for the path without it.
+        __builtin_amdgcn_sched_barrier(0);
+        load_ids(k0 + CH);
+        __builtin_amdgcn_sched_barrier(0);
+        if (__ballot(!(active && n >= CH)) == 0ull) {
+            // every lane has a full segment: no per-pair branches, so no load is moved
+            // into one (where its wait would cover every load in flight)
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            if (u < n) {
+            for (int u = 0; u < CH; u++) {
                 hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
                 if (ENERGY) lj_acc_energy(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz, en, vir);
                 else lj_acc(sx, sy, sz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, lx, ly, lz);
+            }
+        } else {
+            // a row ends in this segment: every slot is evaluated and a slot past the row's
+            // end keeps the lane's image and sums (selects, the same values as skipping it)
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const bool take = u < n;
+                float mx = sx, my = sy, mz = sz, fx = lx, fy = ly, fz = lz;
+                double e = en, w = vir;
+                hyperpos(q[u].x, q[u].y, q[u].z, mx, my, mz, L, half);
+                if (ENERGY) lj_acc_energy(mx, my, mz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, fx, fy, fz, e, w);
+                else lj_acc(mx, my, mz, q[u].x, q[u].y, q[u].z, pss.x, pss.y, fx, fy, fz);
+                sx = take ? mx : sx;
+                sy = take ? my : sy;
+                sz = take ? mz : sz;
+                lx = take ? fx : lx;
+                ly = take ? fy : ly;
+                lz = take ? fz : lz;
+                if (ENERGY) {
+                    en = take ? e : en;
+                    vir = take ? w : vir;
+                }
             }
         }
         // every lane is done with the stage before the next segment overwrites it
