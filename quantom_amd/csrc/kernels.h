@@ -1244,11 +1244,16 @@ __global__ __launch_bounds__(64) void solvent_listed_split_kernel(StepArgs a, in
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    // lane g of a group reads entries k0 + 4g .. k0 + 4g + 3 of the row (16-byte aligned: the rows
+    // are multiples of 16 ids); past the row's count it re-reads entries 4g .. 4g + 3 (in bounds,
+    // not used).  The next segment's ids are requested after the current segment's positions, so
+    // the waits for those (L2) do not also wait for the ids (HBM).
+    auto load_ids = [&](int k0) {
+        return *reinterpret_cast<const int4*>(row + (k0 < cnt ? k0 : 0) + 4 * g);
+    };
+    int4 nxt = wmax > 0 ? load_ids(0) : make_int4(0, 0, 0, 0);
     for (int k0 = 0; k0 < wmax; k0 += 16) {
-        // lane g of a group reads entries k0 + 4g .. k0 + 4g + 3 of the row (16-byte aligned: the
-        // rows are multiples of 16 ids, and k0 < count <= row length)
-        const int4 ids = k0 < cnt ? *reinterpret_cast<const int4*>(row + k0 + 4 * g) : make_int4(si, si, si, si);
-        st4[lane] = ids;
+        st4[lane] = k0 < cnt ? nxt : make_int4(si, si, si, si);
         wave_sync();
         const int n = min(16, cnt - k0);
         float4 q[16];
@@ -1260,23 +1265,39 @@ __global__ __launch_bounds__(64) void solvent_listed_split_kernel(StepArgs a, in
             q[4 * b + 2] = a.spos[4 * b + 2 < n ? i4.z : si];
             q[4 * b + 3] = a.spos[4 * b + 3 < n ? i4.w : si];
         }
+        __builtin_amdgcn_sched_barrier(0);
+        nxt = load_ids(k0 + 16);
+        __builtin_amdgcn_sched_barrier(0);
         wave_sync();   // the stage is rewritten by the next segment
+        // FULL: every lane's row has the whole segment (no per-entry conditions, so no load is
+        // moved into a branch); otherwise an entry past the row's end keeps the image (select)
+        auto sums = [&](auto full_tag) {
+            constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            float ex = sx, ey = sy, ez = sz, qx = 0.f, qy = 0.f, qz = 0.f;
+            for (int b = 0; b < 4; b++) {
+                float ex = sx, ey = sy, ez = sz, qx = 0.f, qy = 0.f, qz = 0.f;
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int u = 4 * b + t;
-                if (u < n) hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
-                const bool mine = t == g;   // this lane's pair: the image after its own entry
-                ex = mine ? sx : ex; ey = mine ? sy : ey; ez = mine ? sz : ez;
-                qx = mine ? q[u].x : qx; qy = mine ? q[u].y : qy; qz = mine ? q[u].z : qz;
+                for (int t = 0; t < 4; t++) {
+                    const int u = 4 * b + t;
+                    if (FULL) {
+                        hyperpos(q[u].x, q[u].y, q[u].z, sx, sy, sz, L, half);
+                    } else {
+                        float mx = sx, my = sy, mz = sz;
+                        hyperpos(q[u].x, q[u].y, q[u].z, mx, my, mz, L, half);
+                        sx = u < n ? mx : sx; sy = u < n ? my : sy; sz = u < n ? mz : sz;
+                    }
+                    const bool mine = t == g;   // this lane's pair: the image after its own entry
+                    ex = mine ? sx : ex; ey = mine ? sy : ey; ez = mine ? sz : ez;
+                    qx = mine ? q[u].x : qx; qy = mine ? q[u].y : qy; qz = mine ? q[u].z : qz;
+                }
+                if (FULL || 4 * b + g < n) {
+                    if (ENERGY) lj_acc_energy(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz, en, vir);
+                    else lj_acc(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz);
+                }
             }
-            if (4 * b + g < n) {
-                if (ENERGY) lj_acc_energy(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz, en, vir);
-                else lj_acc(ex, ey, ez, qx, qy, qz, pss.x, pss.y, lx, ly, lz);
-            }
-        }
+        };
+        if (__ballot(!(active && n >= 16)) == 0ull) sums(std::true_type{});
+        else sums(std::false_type{});
     }
     // the group's partial sums in lane order
     const int g0 = grp * kSplitLanes;
