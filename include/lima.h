@@ -258,13 +258,15 @@ int lima_time_kernels(lima_engine* engine, int32_t reps, double* out8);
  * reference (its Engine::timings, Engine.cuh:123, are host timers per kernel group). */
 int lima_set_kernel_clock(lima_engine* engine, int32_t on);
 int lima_get_kernel_clock(lima_engine* engine, double* total_us, int64_t* launches);
-/* Launch shape chosen for this engine (see lima_autotune), up to eleven values
+/* Launch shape chosen for this engine (see lima_autotune), up to twelve values
    (the first n_out are written): solvents per lane of the all-pairs kernel, its
    target chunk size, its work items, the most chunks of one solvent tile, graph
    replay on/off, solvent cells, solvent tiles, systems, the grid of the listed
-   compound kernel, lanes per solvent of the listed solvent kernel, and the form
-   of the listed compound kernel for compounds of at most 64 particles: 1 packed,
-   2 (FP32) plain with each item's sources split over both wavefronts, else 0. */
+   compound kernel, lanes per solvent of the listed solvent kernel, the form of
+   the listed compound kernel for compounds of at most 64 particles (1 packed,
+   2 (FP32) plain with each item's sources split over both wavefronts, else 0),
+   and 1 where the thermostat runs in the integrators' last blocks instead of a
+   kernel of its own (same sums), else 0. */
 int lima_get_config(lima_engine* engine, int32_t* out, int32_t n_out);
 /* Picks the launch shape (solvents per lane and chunk size of the all-pairs
    kernel, grid of the listed compound kernel, and, in FP32, one or four lanes

@@ -104,6 +104,7 @@ struct LaunchConfig {
     int listed_lanes = 1;         // lanes per solvent of the listed solvent kernel (1, or 4 in FP32)
     int listed_packed = 1;        // compounds of <= 64 particles: the packed listed kernel (1), the plain one (0),
                                   // or (FP32) the plain one with each small item's sources split over both waves (2)
+    int fused_thermostat = 0;     // 1: the thermostat runs in the integrators' last blocks (thermostat_if_last)
 };
 
 // Device buffers of lima_adjoint_run / lima_record_frames, grown on demand and
@@ -185,6 +186,9 @@ struct lima_engine {
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
     DBuf<long long> step_count;   // per replica: steps done (the reference cadence thermostat reads it)
+    DBuf<int> thermo_cnt, thermo_total;   // the thermostat in the integrators' last blocks (thermostat_if_last)
+    bool thermo_ok = false;       // every system has integrator blocks for it
+    bool thermo_fused() const { return thermo_ok && cfg.fused_thermostat; }
     // per-step logs (lima_step_logged): replica, slots per compound, ring length and rings
     int log_rep = -1, log_slots = 0, log_ring = 1;
     bool log_train = false;
@@ -316,6 +320,13 @@ struct lima_engine {
         a.log_slots = log_slots;
         a.log_ring = log_ring;
         a.step_count = step_count.p;
+        a.thermo_cnt = thermo_fused() ? thermo_cnt.p : nullptr;
+        a.thermo_total = thermo_total.p;
+        a.lambda_next = lambda.p;
+        a.lambda_applied = lambda_applied.p;
+        a.temperature = temperature.p;
+        a.step_count_w = step_count.p;
+        a.thermo_mode = params.thermostat;
         a.kclock = kclock_on && kclock_step ? kclock.p : nullptr;
         a.log_data = log_data.p;
         a.train_data = log_rep >= 0 && log_train ? train_data.p : nullptr;
@@ -498,7 +509,8 @@ struct lima_engine {
         // with solvents disabled the solvent state is carried across the ping-pong unchanged
         if (!params.enable_solvents && NS > 0)
             HIP_TRY(hipMemcpyAsync(spos[parity ^ 1].p, spos[parity].p, NS * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-        thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
+        if (!thermo_fused())
+            thermostat_kernel<<<R, 64, 0, stream>>>(reps.p, R, ke_comp.p, ke_stile.p, lambda.p, lambda_applied.p,
                                                 temperature.p, step_count.p, params.thermostat,
                                                 kclock_on && kclock_step ? kclock.p : nullptr);
         if (dev_step >= 0) dev_step++;   // the thermostat kernel advances the device step counters
@@ -1015,6 +1027,23 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
     E->ke_stile.alloc(std::max(E->n_stiles, 1)); E->ke_stile.zero();
     E->temperature.alloc(nsys); E->temperature.zero();
     E->step_count.alloc(nsys); E->step_count.zero();
+    {
+        // the thermostat runs in the integrators' last block of each system when every system has
+        // integrator blocks of the kind that runs after the all-pairs kernel (its solvent tiles,
+        // when solvents are on: the thermostat of system 0 also closes that kernel's clock span)
+        const bool solv = E->params.enable_solvents && E->NS > 0;
+        std::vector<int> total(nsys);
+        bool ok = true;
+        for (int r = 0; r < nsys; r++) {
+            const ReplicaInfo& ri = E->reps_h[r];
+            const int tiles = solv ? ri.stile_end - ri.stile_begin : 0;
+            total[r] = (E->NC > 0 ? ri.comp_end - ri.comp_begin : 0) + tiles;
+            ok = ok && total[r] > 0 && (!solv || tiles > 0);
+        }
+        E->thermo_total.upload(total);
+        E->thermo_cnt.alloc(nsys); E->thermo_cnt.zero();
+        E->thermo_ok = ok;
+    }
     E->energy.alloc((size_t)nsys * kEnergyTerms); E->energy.zero();
     E->err.alloc(4); E->err.zero();
     E->cell_cnt.alloc(E->n_cells); E->cell_start.alloc(E->n_cells + 1); E->cell_cursor.alloc(E->n_cells);
@@ -1670,19 +1699,21 @@ void load_tune_file() {
     std::ifstream f(path);
     std::string line;
     while (std::getline(f, line)) {
-        // key spt chunk_particles listed_grid [listed_lanes [listed_packed]]
+        // key spt chunk_particles listed_grid [listed_lanes [listed_packed [fused_thermostat]]]
         std::istringstream is(line);
         std::string key;
         LaunchConfig c;
         if (!(is >> key >> c.spt >> c.chunk_particles >> c.listed_grid)) continue;
         if (!(is >> c.listed_lanes)) c.listed_lanes = 1;
         if (!(is >> c.listed_packed)) c.listed_packed = 1;
+        if (!(is >> c.fused_thermostat)) c.fused_thermostat = 0;
         // only shapes the autotuner itself can pick: any other line is skipped
         const bool ok = (c.spt == 1 || c.spt == 2 || c.spt == 4) &&
                         (c.chunk_particles == 256 || c.chunk_particles == 512 || c.chunk_particles == 1024 ||
                          c.chunk_particles == 2048) && c.chunk_particles <= kChunkParticles &&
                         (c.listed_grid == 1024 || c.listed_grid == 2048 || c.listed_grid == 4096 || c.listed_grid == 8192) &&
-                        (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes) && (c.listed_packed == 0 || c.listed_packed == 1 || c.listed_packed == 2);
+                        (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes) && (c.listed_packed == 0 || c.listed_packed == 1 || c.listed_packed == 2) &&
+                        (c.fused_thermostat == 0 || c.fused_thermostat == 1);
         if (!ok) continue;
         g_tune_cache[key] = c;
     }
@@ -1693,7 +1724,7 @@ void save_tune_entry(const std::string& key, const LaunchConfig& c) {
     if (!path) return;
     std::ofstream f(path, std::ios::app);
     f << key << " " << c.spt << " " << c.chunk_particles << " " << c.listed_grid << " " << c.listed_lanes << " "
-      << c.listed_packed << "\n";
+      << c.listed_packed << " " << c.fused_thermostat << "\n";
 }
 
 // one system's energy record from the device sums (half of every directed-pair LJ sum)
@@ -2236,12 +2267,13 @@ int lima_time_kernels(lima_engine* E, int32_t reps, double* out8) {
 int lima_get_config(lima_engine* E, int32_t* out, int32_t n_out) {
     return guarded(E, [&] {
         if (!out || n_out < 0) throw LimaError(LIMA_E_ARG, "null output or negative count");
-        const int32_t v[11] = {E->cfg.spt, E->cfg.chunk_particles, (int32_t)E->scwork_h.size(), E->kmax, E->cfg.use_graph,
+        const int32_t v[12] = {E->cfg.spt, E->cfg.chunk_particles, (int32_t)E->scwork_h.size(), E->kmax, E->cfg.use_graph,
                                E->n_cells, E->n_stiles, E->R, E->cfg.listed_grid > 0 ? E->cfg.listed_grid : 4096,
                                E->params.accumulation == LIMA_ACCUM_FIXED64 ? 1 : E->cfg.listed_lanes,
                                !E->pack_ppl() ? 0 : E->cfg.listed_packed == 2 && E->params.accumulation != LIMA_ACCUM_FIXED64 ? 2
-                                                  : E->cfg.listed_packed == 1 ? 1 : 0};
-        for (int i = 0; i < std::min<int32_t>(n_out, 11); i++) out[i] = v[i];
+                                                  : E->cfg.listed_packed == 1 ? 1 : 0,
+                               E->thermo_fused() ? 1 : 0};
+        for (int i = 0; i < std::min<int32_t>(n_out, 12); i++) out[i] = v[i];
         return LIMA_OK;
     });
 }
@@ -2477,6 +2509,20 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 }
                 best.listed_lanes = lmin[1] < lmin[0] ? kSplitLanes : 1;
                 E->cfg.listed_lanes = best.listed_lanes;
+            }
+            if (E->thermo_ok) {
+                // the thermostat as its own kernel after the integrators, or in their last blocks: that
+                // saves a launch, but each integrator block then releases its partial at device scope,
+                // which writes back its XCD's L2 while other kernels may still run
+                double fmin[2] = {1e30, 1e30};
+                for (int sweep = 0; sweep < kSweeps; sweep++) {
+                    for (int j = 0; j < 2; j++) {
+                        E->cfg.fused_thermostat = j;
+                        fmin[j] = std::min(fmin[j], trial());
+                    }
+                }
+                best.fused_thermostat = fmin[1] < fmin[0] ? 1 : 0;
+                E->cfg.fused_thermostat = best.fused_thermostat;
             }
             sweep_all_pairs();
             (void)hipEventDestroy(e0);

@@ -95,6 +95,15 @@ struct StepArgs {
     int log_slots;                // particle slots per compound (the reference's MAX_COMPOUND_PARTICLES)
     int log_ring;                 // steps the device rings hold; step k writes slot k % log_ring
     const long long* step_count;  // [R] steps done before this one
+    // the step's thermostat, run by the last integrator block of each replica where thermo_cnt is
+    // set (otherwise by thermostat_kernel after the integrators)
+    int* thermo_cnt;              // [R] integrator blocks of the replica done this step (back to 0 after)
+    const int* thermo_total;      // [R] its integrator blocks
+    float* lambda_next;           // [R] the lambda array itself (the thermostat writes the next step's)
+    float* lambda_applied;        // [R]
+    double* temperature;          // [R]
+    long long* step_count_w;      // [R] the step counters (the thermostat advances them)
+    int thermo_mode;
     unsigned long long* kclock;   // [4] span of the step's solvent_chain_kernel and its running sum, or null
     float* log_data;              // [log_ring][10]
     float* train_data;            // [log_ring][compounds of log_rep][log_slots][3 x Float3], or null
@@ -914,6 +923,99 @@ __global__ __launch_bounds__(BLOCK) void compound_local_kernel(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Thermostat (Engine::handleBoxtemp, Engine.cu:362-393, made a device FP64
+// reduction).  One wavefront per replica, fixed summation order.
+//   mode 1 (Berendsen): T = sum m|dx|^2 / dt^2 / (3 N R) every step;
+//     lambda = sqrt(1 + dt/tau (T0/T - 1)), clamped to [0.9, 1.1], T=0 guard.
+//   mode 2 (the reference's cadence, Engine.cu:49-75, 314-393): after step k,
+//     with s = k + 1 steps done, s % 40 == 0 evaluates T from the per-particle
+//     FP32 kinetic energies of the positions two steps apart,
+//     T = (float)(sum / N * 2 / (3 * 8.3145)), and, when s > 20, sets
+//     lambda = clamp((float)T0 / T, 0.9, 1.1) for step s only; every other
+//     step runs with lambda = 1.  step_count[r] counts the steps done.
+// ---------------------------------------------------------------------------
+constexpr int kThermostatEvery = 40;   // STEPS_PER_THERMOSTAT (Constants.cuh:82)
+
+// One replica's thermostat by one wavefront (lane = its lane index).
+LIMA_DEV void thermostat_replica(int r, const ReplicaInfo& rp, int lane, const double* ke_comp, const double* ke_stile,
+                                 float* lambda, float* lambda_applied, double* temperature, long long* step_count,
+                                 int mode, unsigned long long* kclock) {
+    double s = 0.0;
+    for (int c = rp.comp_begin + lane; c < rp.comp_end; c += 64) s += ke_comp[c];
+    for (int t = rp.stile_begin + lane; t < rp.stile_end; t += 64) s += ke_stile[t];
+    s = wave_sum(s);
+    if (lane == 0) {
+        lambda_applied[r] = lambda[r];
+        const long long done = step_count[r] + 1;
+        step_count[r] = done;
+        if (kclock && r == 0) {   // this step's span of the all-pairs kernel
+            const unsigned long long b = kclock[0], e = kclock[1];
+            if (b != ~0ull && e > b) {
+                kclock[2] += e - b;
+                kclock[3] += 1;
+            }
+            kclock[0] = ~0ull;
+            kclock[1] = 0ull;
+        }
+        const double ntot = (double)((rp.chain_end - rp.chain_begin) + (rp.solv_end - rp.solv_begin));
+        float lam = 1.0f;
+        if (mode == 2) {
+            if (done % kThermostatEvery == 0) {
+                const double avg = ntot > 0 ? s / ntot : 0.0;
+                float temp = (float)(avg * 2.0f / (3.0f * 8.3145));
+                temp = temp == 0.f ? 1.f : temp;
+                temperature[r] = temp;
+                float sc = rp.T0 / temp;
+                sc = fminf(sc, 1.1f);
+                sc = fmaxf(sc, 0.9f);
+                if (done > 20) lam = sc;
+            }
+        } else {
+            const double temp = ntot > 0 ? s / (rp.dt * rp.dt) / (3.0 * ntot * 8.3145) : 0.0;
+            temperature[r] = temp;
+            if (mode == 1) {
+                double t = temp == 0.0 ? 1.0 : temp;
+                double l = sqrt(1.0 + rp.dt / rp.tau * ((double)rp.T0 / t - 1.0));
+                l = l < 1.1 ? l : 1.1;
+                l = l > 0.9 ? l : 0.9;
+                lam = (float)l;
+            }
+        }
+        lambda[r] = lam;
+    }
+}
+
+__global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps, int n_rep, const double* ke_comp,
+                                                      const double* ke_stile, float* lambda, float* lambda_applied,
+                                                      double* temperature, long long* step_count, int mode,
+                                                      unsigned long long* kclock) {
+    const int r = blockIdx.x;
+    if (r >= n_rep) return;
+    thermostat_replica(r, reps[r], threadIdx.x, ke_comp, ke_stile, lambda, lambda_applied, temperature, step_count, mode,
+                       kclock);
+}
+
+// The thermostat at the end of an integrator block: the replica's last block to finish runs
+// it (a count per replica, no waiting), so no separate launch follows the integrators.  The
+// release fence orders this block's kinetic-energy partial before its count; the last block's
+// acquire fence makes every partial of the replica visible.  The sums are thermostat_kernel's.
+LIMA_DEV void thermostat_if_last(const StepArgs& a, int r) {
+    if (!a.thermo_cnt) return;
+    __shared__ int last;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.thermo_cnt + r, 1) == a.thermo_total[r] - 1;
+    __syncthreads();
+    if (last) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (threadIdx.x < 64)
+            thermostat_replica(r, a.reps[r], threadIdx.x, a.ke_comp, a.ke_stile, a.lambda_next, a.lambda_applied,
+                               a.temperature, a.step_count_w, a.thermo_mode, a.kclock);
+        if (threadIdx.x == 0) a.thermo_cnt[r] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // compoundKernel remainder (Engine.cu:939-1056): the local sum plus the listed
 // partials in list order, integration, PBC.  ADVANCE=false computes forces
 // (and the kinetic energy) without changing the state.
@@ -1034,6 +1136,7 @@ __global__ __launch_bounds__(128) void compound_integrate_kernel(StepArgs a) {
         if ((p & 63) == 0) red[p >> 6] = ks;
         __syncthreads();
         if (p == 0) a.ke_comp[c] = red[0] + (blockDim.x > 64 ? red[1] : 0.0);
+        thermostat_if_last(a, r);
     }
     if (ENERGY) {
         double ks = wave_sum(ke);
@@ -1396,6 +1499,7 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
             for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
             a.ke_stile[tile.w] = tot;
         }
+        thermostat_if_last(a, r);
     }
     if (ENERGY) {
         double ks = wave_sum(ke);
@@ -1407,72 +1511,6 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
             for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
             atomicAdd(a.energy + r * kEnergyTerms + kE_kin, tot);
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Thermostat (Engine::handleBoxtemp, Engine.cu:362-393, made a device FP64
-// reduction).  One wavefront per replica, fixed summation order.
-//   mode 1 (Berendsen): T = sum m|dx|^2 / dt^2 / (3 N R) every step;
-//     lambda = sqrt(1 + dt/tau (T0/T - 1)), clamped to [0.9, 1.1], T=0 guard.
-//   mode 2 (the reference's cadence, Engine.cu:49-75, 314-393): after step k,
-//     with s = k + 1 steps done, s % 40 == 0 evaluates T from the per-particle
-//     FP32 kinetic energies of the positions two steps apart,
-//     T = (float)(sum / N * 2 / (3 * 8.3145)), and, when s > 20, sets
-//     lambda = clamp((float)T0 / T, 0.9, 1.1) for step s only; every other
-//     step runs with lambda = 1.  step_count[r] counts the steps done.
-// ---------------------------------------------------------------------------
-constexpr int kThermostatEvery = 40;   // STEPS_PER_THERMOSTAT (Constants.cuh:82)
-
-__global__ __launch_bounds__(64) void thermostat_kernel(const ReplicaInfo* reps, int n_rep, const double* ke_comp,
-                                                      const double* ke_stile, float* lambda, float* lambda_applied,
-                                                      double* temperature, long long* step_count, int mode,
-                                                      unsigned long long* kclock) {
-    const int r = blockIdx.x;
-    if (r >= n_rep) return;
-    const ReplicaInfo rp = reps[r];
-    double s = 0.0;
-    for (int c = rp.comp_begin + threadIdx.x; c < rp.comp_end; c += 64) s += ke_comp[c];
-    for (int t = rp.stile_begin + threadIdx.x; t < rp.stile_end; t += 64) s += ke_stile[t];
-    s = wave_sum(s);
-    if (threadIdx.x == 0) {
-        lambda_applied[r] = lambda[r];
-        const long long done = step_count[r] + 1;
-        step_count[r] = done;
-        if (kclock && r == 0) {   // this step's span of the all-pairs kernel
-            const unsigned long long b = kclock[0], e = kclock[1];
-            if (b != ~0ull && e > b) {
-                kclock[2] += e - b;
-                kclock[3] += 1;
-            }
-            kclock[0] = ~0ull;
-            kclock[1] = 0ull;
-        }
-        const double ntot = (double)((rp.chain_end - rp.chain_begin) + (rp.solv_end - rp.solv_begin));
-        float lam = 1.0f;
-        if (mode == 2) {
-            if (done % kThermostatEvery == 0) {
-                const double avg = ntot > 0 ? s / ntot : 0.0;
-                float temp = (float)(avg * 2.0f / (3.0f * 8.3145));
-                temp = temp == 0.f ? 1.f : temp;
-                temperature[r] = temp;
-                float sc = rp.T0 / temp;
-                sc = fminf(sc, 1.1f);
-                sc = fmaxf(sc, 0.9f);
-                if (done > 20) lam = sc;
-            }
-        } else {
-            const double temp = ntot > 0 ? s / (rp.dt * rp.dt) / (3.0 * ntot * 8.3145) : 0.0;
-            temperature[r] = temp;
-            if (mode == 1) {
-                double t = temp == 0.0 ? 1.0 : temp;
-                double l = sqrt(1.0 + rp.dt / rp.tau * ((double)rp.T0 / t - 1.0));
-                l = l < 1.1 ? l : 1.1;
-                l = l > 0.9 ? l : 0.9;
-                lam = (float)l;
-            }
-        }
-        lambda[r] = lam;
     }
 }
 

@@ -306,10 +306,11 @@ class Engine:
         return us.value, n.value
 
     def config(self) -> dict:
-        c = (C.c_int32 * 11)()
-        self._check(self.L.lima_get_config(self.h, c, 11))
+        c = (C.c_int32 * 12)()
+        self._check(self.L.lima_get_config(self.h, c, 12))
         keys = ("solvents_per_thread", "chunk_particles", "solvent_chain_items", "max_chunks", "use_graph",
-                "n_cells", "n_solvent_tiles", "n_replicas", "listed_grid", "listed_lanes", "listed_packed")
+                "n_cells", "n_solvent_tiles", "n_replicas", "listed_grid", "listed_lanes", "listed_packed",
+                "fused_thermostat")
         return dict(zip(keys, list(c)))
 
     def autotune(self, trial_steps: int = 20) -> dict:

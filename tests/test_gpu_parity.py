@@ -397,6 +397,46 @@ def test_four_lanes_per_solvent_listed_form(small_system, monkeypatch, tmp_path)
     one.close()
 
 
+def tune_key(systems, accumulation):
+    """The autotune cache key of a batch: the histogram of (chain / 128, solvents / 512) shapes."""
+    from collections import Counter
+    hist = Counter(((s.n_chain + 64) // 128, (s.n_solvents + 256) // 512) for s in systems)
+    return f"acc{accumulation}_sol1" + "".join(f"_{c}x{v}:{n}" for (c, v), n in sorted(hist.items()))
+
+
+@pytest.mark.parametrize("mode", ["berendsen", "reference"])
+def test_thermostat_in_the_integrators_bit_equal(small_system, tiny_system, mode, monkeypatch, tmp_path):
+    """The thermostat run by each system's last integrator block (an autotuner choice) gives the
+    same temperatures, scalars and trajectories, bit for bit, as its own kernel after the
+    integrators: two systems, 100 steps (the reference cadence rescales at steps 40 and 80)."""
+    from quantom_amd.abi import THERMOSTAT_BERENDSEN, THERMOSTAT_REFERENCE, RunParams
+    from quantom_amd.engine import Engine
+    print(f"case: thermostat in the integrators, {mode}", flush=True)
+    syss = [small_system, tiny_system]
+    p = RunParams(nlist_interval=10,
+                  thermostat=THERMOSTAT_BERENDSEN if mode == "berendsen" else THERMOSTAT_REFERENCE)
+    engines = []
+    for fused in (1, 0):   # the same launch shape otherwise
+        path = tmp_path / f"tune{fused}.txt"
+        path.write_text(f"{tune_key(syss, 0)} 2 1024 4096 1 1 {fused}\n")
+        monkeypatch.setenv("LIMA_AUTOTUNE_CACHE", str(path))
+        e = Engine(syss, p)
+        assert e.autotune(10)["fused_thermostat"] == fused
+        engines.append(e)
+    fused, own = engines
+    for _ in range(5):
+        fused.step(20)
+        own.step(20)
+        for x, y in zip(fused.get_state(), own.get_state()):
+            assert np.array_equal(x, y)
+        for a, b in zip(fused.energies(), own.energies()):
+            # the energy record's kinetic term is an FP64 sum of atomic adds (order varies)
+            assert a["temperature"] == pytest.approx(b["temperature"], rel=1e-12)
+            assert a["thermostat_scalar"] == b["thermostat_scalar"]
+    fused.close()
+    own.close()
+
+
 def test_listed_compound_forms(tiny_system, monkeypatch, tmp_path):
     """Compounds of at most 64 particles: the packed listed kernel (the default), the plain one and,
     in FP32, the plain one with each item's sources split over both wavefronts (autotuner choices).
