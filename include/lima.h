@@ -265,8 +265,9 @@ int lima_get_kernel_clock(lima_engine* engine, double* total_us, int64_t* launch
    compound kernel, lanes per solvent of the listed solvent kernel, the form of
    the listed compound kernel for compounds of at most 64 particles (1 packed,
    2 (FP32) plain with each item's sources split over both wavefronts, else 0),
-   and 1 where the thermostat runs in the integrators' last blocks instead of a
-   kernel of its own (same sums), else 0. */
+   and where the thermostat runs: 0 a kernel of its own after the integrators,
+   1 the last of both integrators' blocks of each system, 2 the last compound
+   integrator block, run after the solvent integrator (same sums in all). */
 int lima_get_config(lima_engine* engine, int32_t* out, int32_t n_out);
 /* Picks the launch shape (solvents per lane and chunk size of the all-pairs
    kernel, grid of the listed compound kernel, and, in FP32, one or four lanes

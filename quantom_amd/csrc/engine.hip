@@ -104,7 +104,8 @@ struct LaunchConfig {
     int listed_lanes = 1;         // lanes per solvent of the listed solvent kernel (1, or 4 in FP32)
     int listed_packed = 1;        // compounds of <= 64 particles: the packed listed kernel (1), the plain one (0),
                                   // or (FP32) the plain one with each small item's sources split over both waves (2)
-    int fused_thermostat = 0;     // 1: the thermostat runs in the integrators' last blocks (thermostat_if_last)
+    int fused_thermostat = 0;     // the thermostat in the integrators' last blocks (thermostat_if_last): 1 both
+                                  // integrators count, 2 the compound one only, run after the solvent one
 };
 
 // Device buffers of lima_adjoint_run / lima_record_frames, grown on demand and
@@ -186,9 +187,13 @@ struct lima_engine {
     DBuf<float> lambda, lambda_applied;
     DBuf<double> ke_comp, ke_stile, temperature, energy;
     DBuf<long long> step_count;   // per replica: steps done (the reference cadence thermostat reads it)
-    DBuf<int> thermo_cnt, thermo_total;   // the thermostat in the integrators' last blocks (thermostat_if_last)
-    bool thermo_ok = false;       // every system has integrator blocks for it
-    bool thermo_fused() const { return thermo_ok && cfg.fused_thermostat; }
+    DBuf<int> thermo_cnt, thermo_total, thermo_total_comp;   // the thermostat in the integrators' last blocks
+    bool thermo_ok = false;       // every system has integrator blocks for it (form 1)
+    bool thermo_ok_comp = false;  // every system has compounds (form 2)
+    int thermo_form() const {
+        return cfg.fused_thermostat == 1 && thermo_ok ? 1 : cfg.fused_thermostat == 2 && thermo_ok_comp ? 2 : 0;
+    }
+    bool thermo_fused() const { return thermo_form() != 0; }
     // per-step logs (lima_step_logged): replica, slots per compound, ring length and rings
     int log_rep = -1, log_slots = 0, log_ring = 1;
     bool log_train = false;
@@ -209,7 +214,7 @@ struct lima_engine {
     // s2:     solvent listed LJ
     // s3:     bridge -> compound local forces
     hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_bridge = nullptr, ev_ss = nullptr, ev_c = nullptr, ev_si = nullptr;
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;   // lima_step's device timer
     // Captured launch graphs: one whole rebuild cycle (replayed from its start), and one step
     // of each kind [parity][rebuild follows][clocked] for steps that do not start a cycle.
@@ -232,7 +237,7 @@ struct lima_engine {
 
     ~lima_engine() {
         drop_graphs();
-        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c, ev_t0, ev_t1})
+        for (hipEvent_t e : {ev_fork, ev_bridge, ev_ss, ev_c, ev_si, ev_t0, ev_t1})
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t s : {stream, s1, s2, s3})
             if (s) (void)hipStreamDestroy(s);
@@ -243,7 +248,7 @@ struct lima_engine {
         HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c})
+        for (hipEvent_t* e : {&ev_fork, &ev_bridge, &ev_ss, &ev_c, &ev_si})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
         HIP_TRY(hipEventCreate(&ev_t0));
         HIP_TRY(hipEventCreate(&ev_t1));
@@ -321,7 +326,8 @@ struct lima_engine {
         a.log_ring = log_ring;
         a.step_count = step_count.p;
         a.thermo_cnt = thermo_fused() ? thermo_cnt.p : nullptr;
-        a.thermo_total = thermo_total.p;
+        a.thermo_total = thermo_form() == 2 ? thermo_total_comp.p : thermo_total.p;
+        a.thermo_in_solv = thermo_form() == 1 ? 1 : 0;
         a.lambda_next = lambda.p;
         a.lambda_applied = lambda_applied.p;
         a.temperature = temperature.p;
@@ -488,6 +494,19 @@ struct lima_engine {
         if (NC > 0) launch_compound_local<FIXED, ENERGY>(a, s3);
         HIP_TRY(hipEventRecord(ev_bridge, s3));
         HIP_TRY(hipStreamWaitEvent(s1, ev_bridge, 0));
+        if (ADVANCE && thermo_form() == 2) {
+            // the thermostat in the compound integrator's last blocks: the solvent integrator (and
+            // its kinetic-energy partials) comes first, so the compound blocks' release fences fall
+            // at the end of the step
+            HIP_TRY(hipStreamWaitEvent(stream, ev_ss, 0));
+            if (solv) solvent_integrate_kernel<ADVANCE, FIXED, ENERGY><<<n_stiles, 256, 0, stream>>>(a);
+            HIP_TRY(hipEventRecord(ev_si, stream));
+            HIP_TRY(hipStreamWaitEvent(s1, ev_si, 0));
+            compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, s1>>>(a);
+            HIP_TRY(hipEventRecord(ev_c, s1));
+            HIP_TRY(hipStreamWaitEvent(stream, ev_c, 0));
+            return;
+        }
         if (NC > 0) compound_integrate_kernel<ADVANCE, FIXED, ENERGY><<<NC, kCompoundBlock, 0, s1>>>(a);
         HIP_TRY(hipEventRecord(ev_c, s1));
         HIP_TRY(hipStreamWaitEvent(stream, ev_ss, 0));
@@ -1032,15 +1051,19 @@ void validate_and_build(lima_engine* E, const lima_system* sys, int nsys) {
         // integrator blocks of the kind that runs after the all-pairs kernel (its solvent tiles,
         // when solvents are on: the thermostat of system 0 also closes that kernel's clock span)
         const bool solv = E->params.enable_solvents && E->NS > 0;
-        std::vector<int> total(nsys);
-        bool ok = true;
+        std::vector<int> total(nsys), total_comp(nsys);
+        bool ok = true, ok_comp = E->NC > 0;
         for (int r = 0; r < nsys; r++) {
             const ReplicaInfo& ri = E->reps_h[r];
             const int tiles = solv ? ri.stile_end - ri.stile_begin : 0;
             total[r] = (E->NC > 0 ? ri.comp_end - ri.comp_begin : 0) + tiles;
+            total_comp[r] = ri.comp_end - ri.comp_begin;
             ok = ok && total[r] > 0 && (!solv || tiles > 0);
+            ok_comp = ok_comp && total_comp[r] > 0;
         }
         E->thermo_total.upload(total);
+        E->thermo_total_comp.upload(total_comp);
+        E->thermo_ok_comp = ok_comp;
         E->thermo_cnt.alloc(nsys); E->thermo_cnt.zero();
         E->thermo_ok = ok;
     }
@@ -1713,7 +1736,7 @@ void load_tune_file() {
                          c.chunk_particles == 2048) && c.chunk_particles <= kChunkParticles &&
                         (c.listed_grid == 1024 || c.listed_grid == 2048 || c.listed_grid == 4096 || c.listed_grid == 8192) &&
                         (c.listed_lanes == 1 || c.listed_lanes == kSplitLanes) && (c.listed_packed == 0 || c.listed_packed == 1 || c.listed_packed == 2) &&
-                        (c.fused_thermostat == 0 || c.fused_thermostat == 1);
+                        (c.fused_thermostat >= 0 && c.fused_thermostat <= 2);
         if (!ok) continue;
         g_tune_cache[key] = c;
     }
@@ -2272,7 +2295,7 @@ int lima_get_config(lima_engine* E, int32_t* out, int32_t n_out) {
                                E->params.accumulation == LIMA_ACCUM_FIXED64 ? 1 : E->cfg.listed_lanes,
                                !E->pack_ppl() ? 0 : E->cfg.listed_packed == 2 && E->params.accumulation != LIMA_ACCUM_FIXED64 ? 2
                                                   : E->cfg.listed_packed == 1 ? 1 : 0,
-                               E->thermo_fused() ? 1 : 0};
+                               E->thermo_form()};
         for (int i = 0; i < std::min<int32_t>(n_out, 12); i++) out[i] = v[i];
         return LIMA_OK;
     });
@@ -2510,18 +2533,25 @@ int lima_autotune(lima_engine* E, int32_t reps) {
                 best.listed_lanes = lmin[1] < lmin[0] ? kSplitLanes : 1;
                 E->cfg.listed_lanes = best.listed_lanes;
             }
-            if (E->thermo_ok) {
+            if (E->thermo_ok || E->thermo_ok_comp) {
                 // the thermostat as its own kernel after the integrators, or in their last blocks: that
-                // saves a launch, but each integrator block then releases its partial at device scope,
-                // which writes back its XCD's L2 while other kernels may still run
-                double fmin[2] = {1e30, 1e30};
+                // saves a launch, but each counting block then releases its partial at device scope,
+                // which writes back its XCD's L2 while other kernels may still run (form 1, both
+                // integrators count); form 2 counts in the compound integrator only, run after the
+                // solvent one
+                const bool offer[3] = {true, E->thermo_ok, E->thermo_ok_comp};
+                double fmin[3] = {1e30, 1e30, 1e30};
                 for (int sweep = 0; sweep < kSweeps; sweep++) {
-                    for (int j = 0; j < 2; j++) {
+                    for (int j = 0; j < 3; j++) {
+                        if (!offer[j]) continue;
                         E->cfg.fused_thermostat = j;
                         fmin[j] = std::min(fmin[j], trial());
                     }
                 }
-                best.fused_thermostat = fmin[1] < fmin[0] ? 1 : 0;
+                int jb = 0;
+                for (int j = 1; j < 3; j++)
+                    if (fmin[j] < fmin[jb]) jb = j;
+                best.fused_thermostat = jb;
                 E->cfg.fused_thermostat = best.fused_thermostat;
             }
             sweep_all_pairs();

@@ -104,6 +104,7 @@ struct StepArgs {
     double* temperature;          // [R]
     long long* step_count_w;      // [R] the step counters (the thermostat advances them)
     int thermo_mode;
+    int thermo_in_solv;           // 1: the solvent integrator's blocks count too (else it runs first)
     unsigned long long* kclock;   // [4] span of the step's solvent_chain_kernel and its running sum, or null
     float* log_data;              // [log_ring][10]
     float* train_data;            // [log_ring][compounds of log_rep][log_slots][3 x Float3], or null
@@ -1499,7 +1500,7 @@ __global__ __launch_bounds__(256) void solvent_integrate_kernel(StepArgs a) {
             for (int w = 0; w < (int)blockDim.x / 64; w++) tot += red[w];
             a.ke_stile[tile.w] = tot;
         }
-        thermostat_if_last(a, r);
+        if (a.thermo_in_solv) thermostat_if_last(a, r);
     }
     if (ENERGY) {
         double ks = wave_sum(ke);

@@ -404,19 +404,21 @@ def tune_key(systems, accumulation):
     return f"acc{accumulation}_sol1" + "".join(f"_{c}x{v}:{n}" for (c, v), n in sorted(hist.items()))
 
 
+@pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("mode", ["berendsen", "reference"])
-def test_thermostat_in_the_integrators_bit_equal(small_system, tiny_system, mode, monkeypatch, tmp_path):
-    """The thermostat run by each system's last integrator block (an autotuner choice) gives the
-    same temperatures, scalars and trajectories, bit for bit, as its own kernel after the
+def test_thermostat_in_the_integrators_bit_equal(small_system, tiny_system, mode, form, monkeypatch, tmp_path):
+    """The thermostat run by each system's last integrator block (an autotuner choice: both
+    integrators count, or the compound one alone, run after the solvent one) gives the same
+    temperatures, scalars and trajectories, bit for bit, as its own kernel after the
     integrators: two systems, 100 steps (the reference cadence rescales at steps 40 and 80)."""
     from quantom_amd.abi import THERMOSTAT_BERENDSEN, THERMOSTAT_REFERENCE, RunParams
     from quantom_amd.engine import Engine
-    print(f"case: thermostat in the integrators, {mode}", flush=True)
+    print(f"case: thermostat in the integrators, form {form}, {mode}", flush=True)
     syss = [small_system, tiny_system]
     p = RunParams(nlist_interval=10,
                   thermostat=THERMOSTAT_BERENDSEN if mode == "berendsen" else THERMOSTAT_REFERENCE)
     engines = []
-    for fused in (1, 0):   # the same launch shape otherwise
+    for fused in (form, 0):   # the same launch shape otherwise
         path = tmp_path / f"tune{fused}.txt"
         path.write_text(f"{tune_key(syss, 0)} 2 1024 4096 1 1 {fused}\n")
         monkeypatch.setenv("LIMA_AUTOTUNE_CACHE", str(path))
